@@ -1,0 +1,53 @@
+# Builds the reference MSModel sources, where they lie under /root/reference,
+# into oracle/_ref/ (git-ignored): a shared library of the numerical core for
+# the parity tests and the fixture generator, and the reference's own msmain
+# for the CPU baseline.  Nothing is copied from the reference tree: gcc reads
+# the sources in place and writes objects and binaries under oracle/_ref/.
+#
+# Usage (from the repository root):  make -f oracle/ref.mk
+#
+# The reference Makefile links -llapack, which this image lacks; the conda MKL
+# (LP64, Fortran ABI) provides zgetrf_/zgetrs_/zgesv_/zgeev_/zgels_/... instead.
+# -O2 is added (the reference Makefile compiles at -O0).
+REF      ?= /root/reference
+SRC      := $(REF)/src/core
+OUT      := oracle/_ref
+OBJ      := $(OUT)/obj
+MKLDIR   ?= /opt/conda/lib
+CC       := gcc
+CFLAGS   := -O2 -fPIC -fms-extensions -fopenmp -w -I$(SRC)
+LDLIBS   := -lm -L$(MKLDIR) -lmkl_rt -Wl,-rpath,$(MKLDIR)
+
+# Numerical core on the Foldy-Lax path (medium, Green functions, MS matrix,
+# incident wave, root finder) and the helpers they call.
+CORE     := nru_random common_util real_vector_util complex_vector_util \
+            green_bessel scattering_model_util medium_util ms_matrix_util \
+            incident_wave_util domain_util root_finder
+CORE_OBJ := $(patsubst %,$(OBJ)/%.o,$(CORE))
+ALL_SRC  := $(wildcard $(SRC)/*.c)
+ALL_OBJ  := $(patsubst $(SRC)/%.c,$(OBJ)/%.o,$(ALL_SRC))
+
+all: $(OUT)/libmsref.so $(OUT)/libmsref_shim.so $(OUT)/msmain
+
+$(OBJ):
+	mkdir -p $@
+
+$(OBJ)/%.o: $(SRC)/%.c | $(OBJ)
+	$(CC) $(CFLAGS) -c $< -o $@
+
+$(OBJ)/ms_main.o: $(REF)/src/main/ms_main.c | $(OBJ)
+	$(CC) $(CFLAGS) -c $< -o $@
+
+$(OUT)/libmsref.so: $(CORE_OBJ)
+	$(CC) -shared -fopenmp $^ $(LDLIBS) -o $@
+
+$(OUT)/libmsref_shim.so: oracle/ref_shim.c $(OUT)/libmsref.so
+	$(CC) $(CFLAGS) -shared $< -L$(OUT) -lmsref -Wl,-rpath,'$$ORIGIN' $(LDLIBS) -o $@
+
+$(OUT)/msmain: $(ALL_OBJ) $(OBJ)/ms_main.o
+	$(CC) -fopenmp $^ $(LDLIBS) -o $@
+
+clean:
+	rm -rf $(OUT)
+
+.PHONY: all clean
