@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Foldy-Lax path: log-det evaluations per second at N=1000.
+
+Workload (one "step"): the trace-log tr log M(k) / N of the multiple-scattering
+matrix for d=3, N=1000 scatterers uniform in a unit ball (seed 3), hard spheres
+alpha=0.05, at the 64 x 64 = 4096 wave numbers of a complex map over
+Re k in [5, 30] x Im k in [-2, 0] (map order of get_arg_cmap).  Positions and
+the k grid are uploaded once per call (24 KB + 64 KB); every matrix is
+assembled, factored and reduced on the device.
+
+Reported:
+  value          mixed precision (complex64 LU on MFMA, FP64 re-evaluation of
+                 cells with |det| < 1e-6 max|det| or a pivot ratio < 1e-4), evals/s
+  fp64_batched   the same map in complex128, evals/s
+  fp64_plain     one matrix per call in complex128 (the plain FP64 GPU path), evals/s
+  max_abs_dtracelog  max |tracelog_mixed - tracelog_fp64| over the map
+  resonance latency  p50/p99 of a stream of resonance requests (Aberth, mixed)
+  roofline       dominant kernel (complex64 trailing-update GEMM) from HIP events
+  cpu_baseline   the reference's own msmain (gcc -O2, MKL) on a bounded sample
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_MFMA = 157.3e12
+PEAK_FP64_MFMA = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def kgrid(nx, ny, xmin, xmax, ymin, ymax):
+    """Row-major from the top-left corner, as get_arg_cmap (complex_map_util.c:27-31)."""
+    dx, dy = (xmax - xmin) / (nx - 1), (ymax - ymin) / (ny - 1)
+    l = np.arange(nx * ny)
+    return (xmin + (l % nx) * dx) + 1j * (ymax - (l // nx) * dy)
+
+
+def cpu_baseline(n, nx, ny):
+    """Times the reference msmain (oracle/_ref, built from the MSModel sources with MKL) on a k-map sample."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "msmain")
+    if not os.path.exists(exe):
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    tmp = tempfile.mkdtemp(prefix="msbench_")
+    try:
+        conf = os.path.join(tmp, "map.conf")
+        with open(conf, "w") as f:
+            f.write("[medium]\ndimension=3\nnatom=%d\nmodel=hardsphere 0.05\nshape=ball\nratio=1\n\n"
+                    "[kplane]\nnseed=1\nxrange=5:30\nyrange=-2:0\nxsample=%d\nysample=%d\n"
+                    "color=temperature div 5\ntitle=bench\n" % (n, nx, ny))
+        env = dict(os.environ, MKL_THREADING_LAYER="GNU", MKL_NUM_THREADS="1", OMP_NUM_THREADS=str(cores))
+        subprocess.run([exe, "-p", str(cores), conf], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, timeout=300)
+        dat = None
+        for dp, _, files in os.walk(os.path.join(tmp, "out")):
+            for fn in files:
+                if fn.endswith(".dat"):
+                    dat = os.path.join(dp, fn)
+        rt = None
+        for line in open(dat):
+            if line.startswith("realtime="):
+                rt = float(line.split("=")[1])
+        evals = nx * ny
+        return {"value": evals / rt, "unit": "log-det evals/s", "cores": cores, "kind": "reference",
+                "sample": "msmain -p %d (gcc -O2, MKL sequential per thread): d=3 N=%d unit-density ball seed 1, "
+                          "hardsphere 0.05, %dx%d k-map over Re[5,30] x Im[-2,0] = %d log-dets in %.2f s"
+                          % (cores, n, nx, ny, evals, rt)}
+    except Exception as ex:  # the baseline is reported, never fatal to the GPU measurement
+        return {"value": None, "unit": "log-det evals/s", "cores": cores, "kind": "reference",
+                "sample": "failed: %s" % ex}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def resonance_latencies(G, nreq):
+    """A stream of resonance requests: each is an Aberth search with 8 starts on its own medium
+    (d=2, N in [100, 400] log-uniform, unit-density disk, alpha=0.1), mixed precision."""
+    rng = np.random.default_rng(11)
+    lat = []
+    for r in range(nreq):
+        n = int(round(np.exp(rng.uniform(np.log(100), np.log(400)))))
+        med = G.Medium(2, n, G.HARDSPHERE, (0.1,)).fill(100 + r)
+        dom = (1.0, 12.0, -1.0, 0.0)
+        starts = dom[0] + (dom[1] - dom[0]) * rng.uniform(0.05, 0.95, 8) + \
+            1j * (dom[2] + (dom[3] - dom[2]) * rng.uniform(0.05, 0.95, 8))
+        t0 = time.perf_counter()
+        G.resonances(med, starts, dom, 4, 40, 1e-10, G.MIXED)
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat) * 1e3
+    return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1000)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--plain", type=int, default=24, help="cells timed on the one-matrix-per-call FP64 path")
+    ap.add_argument("--requests", type=int, default=12, help="resonance requests for p50/p99")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py: this build runs on one GPU; --gpus %d is not supported (the k points of a map "
+                 "shard across GPUs without exchange, see DESIGN.md)" % args.gpus)
+
+    import msmodel_amd as G
+    G.init()
+    n, nk = args.n, args.grid * args.grid
+    med = G.Medium(3, n, G.HARDSPHERE, (0.05,)).fill(3, radius=1.0)
+    ks = kgrid(args.grid, args.grid, 5.0, 30.0, -2.0, 0.0)
+
+    for _ in range(args.warmup):
+        G.tracelog_batch(med, ks, G.MIXED, 1e-6)
+    G.profile_begin()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out_mixed, flags = G.tracelog_batch(med, ks, G.MIXED, 1e-6)
+    t_mixed = time.perf_counter() - t0
+    prof = G.profile_end()
+    value = args.steps * nk / t_mixed
+
+    # complex128 on the same map, batched
+    t0 = time.perf_counter()
+    out64, _ = G.tracelog_batch(med, ks, G.FP64)
+    t64 = time.perf_counter() - t0
+    fp64_batched = nk / t64
+    # plain FP64 GPU path: one matrix per call
+    sub = ks[:: max(1, nk // args.plain)][: args.plain]
+    G.tracelog_batch(med, sub[:1], G.FP64)
+    t0 = time.perf_counter()
+    for k in sub:
+        G.tracelog_batch(med, np.array([k]), G.FP64)
+    fp64_plain = len(sub) / (time.perf_counter() - t0)
+    # arg det = N Im(tracelog) is defined modulo 2 pi: compare Im wrapped
+    dre = np.abs(out_mixed.real - out64.real)
+    dim = np.abs((n * (out_mixed.imag - out64.imag) + np.pi) % (2 * np.pi) - np.pi) / n
+    dtl = np.hypot(dre, dim)
+    dtl_re = float(dre.max())
+
+    p50, p99 = resonance_latencies(G, args.requests) if args.requests > 0 else (None, None)
+
+    g = prof["gemm"]
+    achieved = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32_MFMA / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_MFMA, "traffic": None,
+                "kernel": "gemm_sub_kernel<float> (trailing update, complex64 MFMA 16x16x4)",
+                "launches": g["launches"], "avg_ms": g["ms"] / max(1, g["launches"])}
+    breakdown = {c: round(v["ms"] / max(1e-9, sum(x["ms"] for x in prof.values())), 4) for c, v in prof.items()}
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(n, 16, 8)
+    line = {
+        "metric": "log-det evals/s at N=1000 (mixed vs FP64, same GPU); p50/p99 resonance latency",
+        "value": value, "unit": "log-det evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64+c128", "data": "synthetic",
+        "config": {"workload": "tracelog map d=3 N=%d unit ball seed 3 hardsphere 0.05, %dx%d k over "
+                               "Re[5,30] x Im[-2,0]" % (n, args.grid, args.grid),
+                   "cells_per_step": nk, "reeval_rel": 1e-6},
+        "fp64_batched_evals_per_s": fp64_batched, "fp64_plain_evals_per_s": fp64_plain,
+        "mixed_over_fp64_plain": value / fp64_plain, "mixed_over_fp64_batched": value / fp64_batched,
+        "reevaluated_cells": int(flags.sum()),
+        "max_abs_dtracelog": float(dtl.max()), "max_abs_dtracelog_re": dtl_re,
+        "resonance_latency_ms": {"p50": p50, "p99": p99, "requests": args.requests},
+        "kernel_time_fraction": breakdown,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,158 @@
+/*
+ * msgpu.h -- C ABI of the MI355X Foldy-Lax library (libmsgpu.so).
+ *
+ * Two groups of entry points:
+ *
+ *  1. Drop-in replacements that keep MSModel's C signatures and semantics
+ *     (host buffers in and out, input matrices destroyed as LAPACK does, LAPACK
+ *     errors printed with the "[LAPACK]" tag and never fatal).  A program that
+ *     links libmsgpu.so ahead of the reference objects
+ *     runs its matrix work on the GPU unchanged.  Each declaration cites the
+ *     reference declaration it replaces (paths relative to the MSModel tree).
+ *
+ *  2. Batched, device-resident entry points (msgpu_*) that the task drivers
+ *     of ms_core.c call with all k points of a map, all Newton starts, or a
+ *     whole psi grid at once.  These are the fast path.
+ *
+ * Complex numbers are C99 "double _Complex" (interleaved re, im FP64), the
+ * reference's dcomplex (dcomplex_type.h:16).  Structures are laid out exactly
+ * as the reference's (same member order, types and enum numbering), under
+ * prefixed names so this header can be included beside the reference headers.
+ */
+#ifndef MSGPU_H
+#define MSGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+#include <complex>
+typedef std::complex<double> msgpu_cplx;   /* same layout and return ABI as double _Complex */
+extern "C" {
+#else
+#include <complex.h>
+typedef double _Complex msgpu_cplx;
+#endif
+
+/* ------------------------------------------------------------------ types */
+/* Urandom, nru_random_type.h:16-21 */
+typedef struct { uint64_t u, v, w, seed; } msgpu_urandom;
+/* ScatteringType, scattering_model_type.h:17 */
+enum { MSGPU_HARDSPHERE = 0, MSGPU_SOFTSPHERE = 1, MSGPU_RESONANT = 2, MSGPU_MAXIMUM = 3 };
+/* ScatteringModel, scattering_model_type.h:22-34 */
+typedef struct {
+    int type;
+    int np;
+    double* p;
+    char dir[50];
+    char name[150];
+    char param_ascii[150];
+    char param_latex[150];
+} msgpu_scmodel;
+/* Shape, medium_type.h:19 */
+enum { MSGPU_CUBE = 0, MSGPU_BALL = 1, MSGPU_LATTICE = 2, MSGPU_GAUSSIAN = 3 };
+/* Medium, medium_type.h:24-36 */
+typedef struct {
+    int d;
+    int n;
+    int ntot;
+    int shape;
+    double ratio;
+    msgpu_scmodel* scmodel;
+    double alpha;
+    msgpu_urandom* rnd;
+    double* pos;
+} msgpu_medium;
+/* IncidentWaveSource / IncidentWave, incident_wave_type.h:17-31 */
+enum { MSGPU_PLANE = 0, MSGPU_SPHERICAL = 1 };
+typedef struct {
+    int source;
+    msgpu_cplx k;
+    char desc_ascii[80];
+    char desc_latex[80];
+} msgpu_iwave;
+
+/* Arithmetic of the batched entry points. */
+enum {
+    MSGPU_FP64 = 0,    /* complex128 everywhere                                      */
+    MSGPU_MIXED = 1,   /* complex64 LU on MFMA; FP64 re-evaluation of flagged cells  */
+    MSGPU_FP32 = 2     /* complex64 LU only (error reported, not corrected)          */
+};
+
+/* ------------------------------------------------- 1. drop-in (reference ABI) */
+/* medium_util.h:55-64 and medium_util.c:36 (init_medium is undeclared there). */
+void   init_medium(msgpu_medium* med, int d, int n, msgpu_scmodel* scmodel, int shape, double ratio);
+void   init_copy_medium(msgpu_medium* src, msgpu_medium* dst);
+void   del_medium(msgpu_medium* med);
+double radius_of_ball(msgpu_medium* med);
+void   fill_medium(msgpu_medium* med, uint64_t seed);
+double atom_distance(msgpu_medium* med, int i, int j);
+msgpu_cplx invf(msgpu_medium* med, msgpu_cplx k);                       /* medium_util.h:64 */
+/* scattering_model_util.h:17 */
+int    init_scmodel(msgpu_scmodel* scmodel, int type, int np, double* p);
+
+/* ms_matrix_util.h:17-30 */
+void       build_ms_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* matrix);
+void       build_normalized_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* matrix);
+void       build_dk_ms_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* matrix);
+void       build_plane_wave(msgpu_medium* med, msgpu_cplx k, double theta, msgpu_cplx* vec);
+msgpu_cplx tracelog_lu(int n, msgpu_cplx* matrix);
+msgpu_cplx dk_log_det_ms(msgpu_medium* med, msgpu_cplx k);
+msgpu_cplx traceinv(int n, msgpu_cplx* matrix);
+int        inverse_iteration(int n, msgpu_cplx* a, msgpu_cplx* vec, msgpu_cplx* lambda, int maxit, double toler,
+                             int verb);
+double     total_cross_section(msgpu_medium* med, double k);
+
+/* incident_wave_util.h:18-19 */
+msgpu_cplx eval_incident_wave(msgpu_iwave* iwave, int d, double* pos);
+msgpu_cplx eval_total_wave(msgpu_iwave* iwave, msgpu_medium* med, msgpu_cplx* a, double* pos);
+
+/* LAPACK Fortran ABI (LP64) as declared at ms_matrix_util.c:137-142 and ms_core.c:47. */
+void zgetrf_(int* m, int* n, msgpu_cplx* a, int* lda, int* ipiv, int* info);
+void zgetrs_(char* trans, int* n, int* nrhs, msgpu_cplx* a, int* lda, int* ipiv, msgpu_cplx* b, int* ldb,
+             int* info);
+void zgesv_(int* n, int* nrhs, msgpu_cplx* a, int* lda, int* ipiv, msgpu_cplx* b, int* ldb, int* info);
+
+/* ---------------------------------------------- 2. batched / device-resident */
+/* Library set-up; called implicitly by every entry point.  Returns 0 on success. */
+int msgpu_init(void);
+/* Last error message of the calling thread ("" if none). */
+const char* msgpu_last_error(void);
+/* Kernel timing (no counterpart in the reference; measurement hook for bench.py):
+ * between begin and end every batched launch is bracketed by HIP events on its own
+ * stream.  end() fills, per category c < ncat (0 panel, 1 swap+trsm, 2 gemm,
+ * 3 assembly, 4 tracelog), the summed milliseconds, algorithmic flops and bytes and
+ * the launch count. */
+int msgpu_profile_begin(void);
+int msgpu_profile_end(double* ms, double* flops, double* bytes, int* count, int ncat);
+
+/* Trace-log of M(k) for nk wave numbers of one medium: out[l] = tracelog_lu(M(ks[l]))
+ * as in run_tracelog_omp (ms_core.c:122-127).  mode = MSGPU_FP64/MIXED/FP32.
+ * For MSGPU_MIXED, cells with N*Re(tracelog) < max_l N*Re(tracelog) + ln(reeval_rel)
+ * (|det| below reeval_rel of the map maximum) and cells whose complex64 factor
+ * has a pivot below pivot_tol*max|U_ii| are recomputed in FP64 (reeval_rel<=0
+ * disables the first test).  flags[l] (may be NULL) = 1 for re-evaluated cells.
+ * Returns 0 on success, <0 on error (message in msgpu_last_error). */
+int msgpu_tracelog_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode,
+                         double reeval_rel, int* flags);
+/* d/dk log det M(k) = tr(M^-1 M') for nk wave numbers (ms_matrix_util.c:242-263). */
+int msgpu_dk_log_det_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode);
+/* Foldy-Lax amplitudes a = M(k)^-1 phi with phi the incident wave at the scatterers
+ * (ms_core.c:616-620).  mode MSGPU_FP64: complex128 LU; MSGPU_MIXED: complex64 LU
+ * with FP64 iterative refinement until ||M a - phi|| <= tol ||phi|| (tol<=0: 1e-12).
+ * Returns the number of refinement steps (>=0) or <0 on error. */
+int msgpu_solve(const msgpu_medium* med, const msgpu_iwave* iwave, msgpu_cplx* a, int mode, double tol);
+/* psi(r) = phi(r) + sum_i a_i G(r|x_i) on an nx x ny grid of the (x0,x1) plane,
+ * row-major from the top-left corner (wave_function_util.c:76-81). */
+int msgpu_wavefunction_grid(const msgpu_medium* med, const msgpu_iwave* iwave, const msgpu_cplx* a, int nx,
+                            int ny, double xmin, double xmax, double ymin, double ymax, msgpu_cplx* psi);
+
+/* Resonances: Maehly-Aberth-Ehrlich on det M(k) (root_finder.c:522-577) with every
+ * live start's f'/f evaluated in one batched launch per sweep.  z[0..*nz) holds the
+ * starts on entry and the converged roots (sorted by real part) on exit.
+ * mode MSGPU_MIXED iterates in complex64 then polishes every root in FP64. */
+int msgpu_resonances(const msgpu_medium* med, double xmin, double xmax, double ymin, double ymax, int* nz,
+                     msgpu_cplx* z, int nexpsup, int maxit, double toler, int mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

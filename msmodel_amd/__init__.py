@@ -1,0 +1,315 @@
+"""msmodel_amd -- MI355X implementation of the MSModel Foldy-Lax hot path.
+
+This package is a thin ctypes binding of libmsgpu.so (HIP kernels for gfx950
+behind a C ABI with the reference's signatures, see include/msgpu.h).  There is
+no CPU fallback: importing the package fails if the library is not built, and
+every entry point raises if the device call fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmsgpu.so")
+
+HARDSPHERE, SOFTSPHERE, RESONANT, MAXIMUM = 0, 1, 2, 3
+CUBE, BALL, LATTICE, GAUSSIAN = 0, 1, 2, 3
+PLANE, SPHERICAL = 0, 1
+FP64, MIXED, FP32 = 0, 1, 2
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("msmodel_amd: %s is not built; run `python msmodel_amd/build.py` "
+                      "(there is no CPU fallback)" % LIB_PATH)
+
+
+class Cplx(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+def _c(z):
+    z = complex(z)
+    return Cplx(z.real, z.imag)
+
+
+def _z(c):
+    return complex(c.re, c.im)
+
+
+class Urandom(ctypes.Structure):
+    _fields_ = [("u", ctypes.c_uint64), ("v", ctypes.c_uint64), ("w", ctypes.c_uint64), ("seed", ctypes.c_uint64)]
+
+
+class ScModel(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("np", ctypes.c_int), ("p", ctypes.POINTER(ctypes.c_double)),
+                ("dir", ctypes.c_char * 50), ("name", ctypes.c_char * 150),
+                ("param_ascii", ctypes.c_char * 150), ("param_latex", ctypes.c_char * 150)]
+
+
+class CMedium(ctypes.Structure):
+    _fields_ = [("d", ctypes.c_int), ("n", ctypes.c_int), ("ntot", ctypes.c_int), ("shape", ctypes.c_int),
+                ("ratio", ctypes.c_double), ("scmodel", ctypes.POINTER(ScModel)), ("alpha", ctypes.c_double),
+                ("rnd", ctypes.POINTER(Urandom)), ("pos", ctypes.POINTER(ctypes.c_double))]
+
+
+class IWave(ctypes.Structure):
+    _fields_ = [("source", ctypes.c_int), ("k", Cplx), ("desc_ascii", ctypes.c_char * 80),
+                ("desc_latex", ctypes.c_char * 80)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_mp = ctypes.POINTER(CMedium)
+
+lib = ctypes.CDLL(LIB_PATH)
+_SIG = {
+    "msgpu_init": (ctypes.c_int, []),
+    "msgpu_last_error": (ctypes.c_char_p, []),
+    "msgpu_profile_begin": (ctypes.c_int, []),
+    "msgpu_profile_end": (ctypes.c_int, [_dp, _dp, _dp, _ip, ctypes.c_int]),
+    "msgpu_tracelog_batch": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, _ip]),
+    "msgpu_dk_log_det_batch": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
+    "msgpu_solve": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_double]),
+    "msgpu_wavefunction_grid": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                               _dp]),
+    "msgpu_resonances": (ctypes.c_int, [_mp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "init_scmodel": (ctypes.c_int, [ctypes.POINTER(ScModel), ctypes.c_int, ctypes.c_int, _dp]),
+    "init_medium": (None, [_mp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ScModel), ctypes.c_int, ctypes.c_double]),
+    "del_medium": (None, [_mp]),
+    "fill_medium": (None, [_mp, ctypes.c_uint64]),
+    "radius_of_ball": (ctypes.c_double, [_mp]),
+    "atom_distance": (ctypes.c_double, [_mp, ctypes.c_int, ctypes.c_int]),
+    "invf": (Cplx, [_mp, Cplx]),
+    "build_ms_matrix": (None, [_mp, Cplx, _dp]),
+    "build_dk_ms_matrix": (None, [_mp, Cplx, _dp]),
+    "build_normalized_matrix": (None, [_mp, Cplx, _dp]),
+    "build_plane_wave": (None, [_mp, Cplx, ctypes.c_double, _dp]),
+    "tracelog_lu": (Cplx, [ctypes.c_int, _dp]),
+    "dk_log_det_ms": (Cplx, [_mp, Cplx]),
+    "traceinv": (Cplx, [ctypes.c_int, _dp]),
+    "inverse_iteration": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "total_cross_section": (ctypes.c_double, [_mp, ctypes.c_double]),
+    "eval_incident_wave": (Cplx, [ctypes.POINTER(IWave), ctypes.c_int, _dp]),
+    "eval_total_wave": (Cplx, [ctypes.POINTER(IWave), _mp, _dp, _dp]),
+    "zgetrf_": (None, [_ip, _ip, _dp, _ip, _ip, _ip]),
+    "zgetrs_": (None, [ctypes.c_char_p, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip]),
+    "zgesv_": (None, [_ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip]),
+}
+for _name, (_res, _args) in _SIG.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = sorted(_SIG)
+
+
+class MsgpuError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise MsgpuError("%s failed (%d): %s" % (what, rc, lib.msgpu_last_error().decode()))
+    return rc
+
+
+def _cp(a):
+    return a.ctypes.data_as(_dp)
+
+
+def init():
+    return _check(lib.msgpu_init(), "msgpu_init")
+
+
+PROFILE_CATEGORIES = ("panel", "swap_trsm", "gemm", "assembly", "tracelog")
+
+
+def profile_begin():
+    _check(lib.msgpu_profile_begin(), "msgpu_profile_begin")
+
+
+def profile_end():
+    nc = len(PROFILE_CATEGORIES)
+    ms, fl, by = np.zeros(nc), np.zeros(nc), np.zeros(nc)
+    cnt = np.zeros(nc, dtype=np.int32)
+    _check(lib.msgpu_profile_end(_cp(ms), _cp(fl), _cp(by), cnt.ctypes.data_as(_ip), nc), "msgpu_profile_end")
+    return {c: {"ms": float(ms[i]), "flops": float(fl[i]), "bytes": float(by[i]), "launches": int(cnt[i])}
+            for i, c in enumerate(PROFILE_CATEGORIES)}
+
+
+class Medium:
+    """A reference-layout Medium owned by libmsgpu (init_medium / fill_medium / del_medium)."""
+
+    def __init__(self, d, n, model=HARDSPHERE, p=(0.1,), shape=BALL, ratio=1.0):
+        self._p = np.ascontiguousarray(np.atleast_1d(np.asarray(p, dtype=np.float64)))
+        if self._p.size == 0:
+            self._p = np.zeros(1)
+        sm = ScModel()
+        npar = {HARDSPHERE: 1, SOFTSPHERE: 1, RESONANT: 2, MAXIMUM: 0}[model]
+        lib.init_scmodel(ctypes.byref(sm), model, npar, _cp(self._p))
+        self.c = CMedium()
+        lib.init_medium(ctypes.byref(self.c), d, n, ctypes.byref(sm), shape, ratio)
+        self.d, self.n = d, n
+
+    def __del__(self):
+        try:
+            lib.del_medium(ctypes.byref(self.c))
+        except Exception:
+            pass
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.c)
+
+    def fill(self, seed, radius=None):
+        """fill_medium(seed); with radius R, positions are then scaled by R/radius_of_ball."""
+        lib.fill_medium(self.ref, seed)
+        if radius is not None:
+            self.set_pos(self.pos * (radius / lib.radius_of_ball(self.ref)))
+        return self
+
+    @property
+    def pos(self):
+        return np.ctypeslib.as_array(self.c.pos, shape=(self.n * self.c.d,)).reshape(self.n, self.c.d).copy()
+
+    def set_pos(self, pos):
+        flat = np.ascontiguousarray(pos, dtype=np.float64).ravel()
+        assert flat.size == self.n * self.c.d
+        ctypes.memmove(self.c.pos, flat.ctypes.data, flat.nbytes)
+        return self
+
+
+def _iwave(k, source):
+    w = IWave()
+    w.source = source
+    w.k = _c(k)
+    return w
+
+
+# ------------------------------------------------------------------ batched entry points
+def tracelog_batch(med, ks, mode=FP64, reeval_rel=1e-6):
+    ks = np.ascontiguousarray(ks, dtype=np.complex128)
+    out = np.zeros_like(ks)
+    flags = np.zeros(ks.size, dtype=np.int32)
+    _check(lib.msgpu_tracelog_batch(med.ref, ks.size, _cp(ks), _cp(out), mode, reeval_rel,
+                                    flags.ctypes.data_as(_ip)), "msgpu_tracelog_batch")
+    return out, flags
+
+
+def dk_log_det_batch(med, ks, mode=FP64):
+    ks = np.ascontiguousarray(ks, dtype=np.complex128)
+    out = np.zeros_like(ks)
+    _check(lib.msgpu_dk_log_det_batch(med.ref, ks.size, _cp(ks), _cp(out), mode), "msgpu_dk_log_det_batch")
+    return out
+
+
+def solve(med, k, source=PLANE, mode=FP64, tol=0.0):
+    a = np.zeros(med.n, dtype=np.complex128)
+    w = _iwave(k, source)
+    steps = _check(lib.msgpu_solve(med.ref, ctypes.byref(w), _cp(a), mode, tol), "msgpu_solve")
+    return a, steps
+
+
+def wavefunction_grid(med, k, a, nx, ny, xmin, xmax, ymin, ymax, source=PLANE):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    psi = np.zeros(nx * ny, dtype=np.complex128)
+    w = _iwave(k, source)
+    _check(lib.msgpu_wavefunction_grid(med.ref, ctypes.byref(w), _cp(a), nx, ny, xmin, xmax, ymin, ymax, _cp(psi)),
+           "msgpu_wavefunction_grid")
+    return psi.reshape(ny, nx)
+
+
+def resonances(med, starts, dom, nexpsup=4, maxit=50, toler=1e-12, mode=FP64):
+    z = np.array(starts, dtype=np.complex128, copy=True)
+    nz = ctypes.c_int(z.size)
+    sweeps = _check(lib.msgpu_resonances(med.ref, dom[0], dom[1], dom[2], dom[3], ctypes.byref(nz), _cp(z), nexpsup,
+                                         maxit, toler, mode), "msgpu_resonances")
+    return z[: nz.value], sweeps
+
+
+# ------------------------------------------------------------------ drop-in entry points
+def build_matrix(med, k, kind="ms"):
+    m = np.zeros((med.n, med.n), dtype=np.complex128, order="F")
+    fn = {"ms": lib.build_ms_matrix, "dk": lib.build_dk_ms_matrix, "normalized": lib.build_normalized_matrix}[kind]
+    fn(med.ref, _c(k), _cp(m))
+    return m
+
+
+def plane_wave(med, k, theta_deg):
+    v = np.zeros(med.n, dtype=np.complex128)
+    lib.build_plane_wave(med.ref, _c(k), theta_deg, _cp(v))
+    return v
+
+
+def tracelog_lu(m):
+    a = np.array(m, dtype=np.complex128, order="F", copy=True)
+    t = _z(lib.tracelog_lu(a.shape[0], _cp(a)))
+    return t, a
+
+
+def dk_log_det_ms(med, k):
+    return _z(lib.dk_log_det_ms(med.ref, _c(k)))
+
+
+def traceinv(m):
+    a = np.array(m, dtype=np.complex128, order="F", copy=True)
+    return _z(lib.traceinv(a.shape[0], _cp(a)))
+
+
+def inverse_iteration(m, vec, maxit=50, toler=1e-10):
+    a = np.array(m, dtype=np.complex128, order="F", copy=True)
+    v = np.array(vec, dtype=np.complex128, copy=True)
+    lam = np.zeros(1, dtype=np.complex128)
+    conv = lib.inverse_iteration(a.shape[0], _cp(a), _cp(v), _cp(lam), maxit, toler, 0)
+    return conv, v, complex(lam[0])
+
+
+def total_cross_section(med, k):
+    return lib.total_cross_section(med.ref, float(k))
+
+
+def eval_total_wave(med, k, a, x, source=PLANE):
+    w = _iwave(k, source)
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return _z(lib.eval_total_wave(ctypes.byref(w), med.ref, _cp(a), _cp(x)))
+
+
+def invf(med, k):
+    return _z(lib.invf(med.ref, _c(k)))
+
+
+def zgesv(a, b):
+    a = np.array(a, dtype=np.complex128, order="F", copy=True)
+    b = np.array(b, dtype=np.complex128, order="F", copy=True)
+    n = ctypes.c_int(a.shape[0])
+    nrhs = ctypes.c_int(1 if b.ndim == 1 else b.shape[1])
+    ipiv = np.zeros(a.shape[0], dtype=np.int32)
+    info = ctypes.c_int(0)
+    lib.zgesv_(ctypes.byref(n), ctypes.byref(nrhs), _cp(a), ctypes.byref(n), ipiv.ctypes.data_as(_ip), _cp(b),
+               ctypes.byref(n), ctypes.byref(info))
+    return b, a, ipiv, info.value
+
+
+def zgetrf(a):
+    a = np.array(a, dtype=np.complex128, order="F", copy=True)
+    n = ctypes.c_int(a.shape[0])
+    ipiv = np.zeros(a.shape[0], dtype=np.int32)
+    info = ctypes.c_int(0)
+    lib.zgetrf_(ctypes.byref(n), ctypes.byref(n), _cp(a), ctypes.byref(n), ipiv.ctypes.data_as(_ip),
+                ctypes.byref(info))
+    return a, ipiv, info.value
+
+
+def zgetrs(lu, ipiv, b):
+    lu = np.asfortranarray(lu, dtype=np.complex128)
+    b = np.array(b, dtype=np.complex128, order="F", copy=True)
+    n = ctypes.c_int(lu.shape[0])
+    nrhs = ctypes.c_int(1 if b.ndim == 1 else b.shape[1])
+    info = ctypes.c_int(0)
+    ip = np.ascontiguousarray(ipiv, dtype=np.int32)
+    lib.zgetrs_(b"N", ctypes.byref(n), ctypes.byref(nrhs), _cp(lu), ctypes.byref(n), ip.ctypes.data_as(_ip), _cp(b),
+                ctypes.byref(n), ctypes.byref(info))
+    return b, info.value

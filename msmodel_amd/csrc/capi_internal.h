@@ -1,0 +1,62 @@
+// capi_internal.h -- declarations shared by the host translation units of
+// libmsgpu.so (medium.cpp, engine.cpp, capi.cpp).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/msgpu.h"
+#include "dmath.h"
+
+#define MSGPU_API __attribute__((visibility("default")))
+
+namespace msg {
+
+// medium.cpp
+double ball_volume(int d);
+double ball_radius(int d, int n);
+void fill_positions(msgpu_medium* med, uint64_t seed);
+ScModel scmodel_of(const msgpu_medium* med);
+
+// What the engine needs to know about a medium.
+struct MediumView {
+    int d = 0, n = 0, np = 0;
+    ScModel m{};
+    const double* pos = nullptr;   // host, row-major [n][d]
+};
+MediumView view_of(const msgpu_medium* med);
+inline int padded(int n) { return (n + 63) / 64 * 64; }
+
+// engine.cpp -- every function returns 0 on success, <0 on error (message via set_error)
+void set_error(const std::string& msg);
+const char* last_error();
+int engine_init();
+int profile_begin();
+int profile_end(double* ms, double* flops, double* bytes, int* count, int ncat);
+
+// Batched trace-log for nk wave numbers; precision 0 = FP64, 1 = FP32.
+// stats (may be null) receives {max log|U_ii|, min log|U_ii|} per cell.
+int tracelog_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, double* stats,
+                    int precision);
+// tr(M^-1 dM/dk) for nk wave numbers.
+int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, int precision);
+// Amplitudes a = M(k)^-1 phi; refine: complex64 LU + FP64 refinement to tol.  Returns #steps or <0.
+int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>* a, int mixed, double tol);
+// psi grid
+int wave_grid(const MediumView& mv, cx<double> k, int source, const cx<double>* a, int nx, int ny, double xmin,
+              double xmax, double ymin, double ymax, cx<double>* psi);
+// Matrix assembly into a host buffer (n x n, ld n): kind 0 M, 1 dM/dk, 2 normalised.
+int build_matrix_host(const MediumView& mv, cx<double> k, int kind, cx<double>* out);
+int plane_wave_host(const MediumView& mv, cx<double> k, double theta_deg, cx<double>* out);
+int incident_host(const MediumView& mv, cx<double> k, int source, cx<double>* out);
+// Host matrix (n x n, ld lda) -> LU on the device -> written back in LAPACK format.
+// Optionally solves nrhs right-hand sides b (ld ldb) in place and returns the tracelog.
+int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, int ldb, cx<double>* tracelog,
+            int* info);
+// Solve with an existing LAPACK-format LU from the host (zgetrs 'N').
+int lu_solve_host(int n, const cx<double>* lu, int lda, const int* ipiv, int nrhs, cx<double>* b, int ldb);
+// psi at arbitrary points (npt points, row-major [npt][d]).
+int wave_points(const MediumView& mv, cx<double> k, int source, const cx<double>* a, int npt, const double* pts,
+                cx<double>* psi);
+
+}  // namespace msg

@@ -1,0 +1,594 @@
+// engine.cpp -- host driver of the device path: device buffer pool, stream
+// leases for concurrent callers, and the batched assembly / LU / solve / psi
+// pipelines built from the kernels of kern_*.hip.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "capi_internal.h"
+#include "kernels.h"
+
+namespace msg {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string t_err;
+void set_error(const std::string& m) { t_err = m; }
+const char* last_error() { return t_err.c_str(); }
+
+struct HipFail : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess)                                                                        \
+            throw HipFail(std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" +   \
+                          std::to_string(__LINE__) + ")");                                           \
+    } while (0)
+
+// ------------------------------------------------------------------ device pool
+// Buffers are recycled by size class (next power of two above 1 MiB granularity)
+// and never returned to the driver, so steady-state calls allocate nothing.
+class Pool {
+    std::mutex mu_;
+    std::multimap<size_t, void*> free_;
+
+public:
+    static size_t klass(size_t b) {
+        size_t c = (size_t)1 << 16;
+        while (c < b) c <<= 1;
+        return c;
+    }
+    void* get(size_t bytes, size_t& cls) {
+        cls = klass(bytes == 0 ? 1 : bytes);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto it = free_.find(cls);
+            if (it != free_.end()) {
+                void* p = it->second;
+                free_.erase(it);
+                return p;
+            }
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, cls);
+        if (e != hipSuccess) {
+            // release cached buffers and retry once
+            std::lock_guard<std::mutex> g(mu_);
+            for (auto& kv : free_) (void)hipFree(kv.second);
+            free_.clear();
+            HIPCHK(hipMalloc(&p, cls));
+        }
+        return p;
+    }
+    void put(void* p, size_t cls) {
+        std::lock_guard<std::mutex> g(mu_);
+        free_.emplace(cls, p);
+    }
+};
+
+static Pool& pool() {
+    static Pool* p = new Pool();
+    return *p;
+}
+
+template <class T> struct DBuf {
+    T* p = nullptr;
+    size_t cls = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { p = (T*)pool().get(sizeof(T) * count, cls); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() {
+        if (p) pool().put(p, cls);
+    }
+};
+
+// ------------------------------------------------------------------ streams
+class StreamPool {
+    std::mutex mu_;
+    std::vector<hipStream_t> free_;
+
+public:
+    hipStream_t get() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            if (!free_.empty()) {
+                hipStream_t s = free_.back();
+                free_.pop_back();
+                return s;
+            }
+        }
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        return s;
+    }
+    void put(hipStream_t s) {
+        std::lock_guard<std::mutex> g(mu_);
+        free_.push_back(s);
+    }
+};
+static StreamPool& streams() {
+    static StreamPool* p = new StreamPool();
+    return *p;
+}
+struct Lease {
+    hipStream_t st;
+    Lease() : st(streams().get()) {}
+    ~Lease() { streams().put(st); }
+    void sync() { HIPCHK(hipStreamSynchronize(st)); }
+};
+
+static std::once_flag g_once;
+static int g_init_rc = -1;
+
+int engine_init() {
+    std::call_once(g_once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+            g_init_rc = -1;
+            return;
+        }
+        if (hipSetDevice(0) != hipSuccess) {
+            g_init_rc = -2;
+            return;
+        }
+        g_init_rc = 0;
+    });
+    if (g_init_rc != 0) set_error("msgpu: no usable HIP device (hipGetDeviceCount/hipSetDevice failed)");
+    return g_init_rc;
+}
+
+template <class F>
+static int guarded(F&& f) {
+    if (engine_init() != 0) return -1;
+    try {
+        return f();
+    } catch (const std::exception& ex) {
+        set_error(std::string("msgpu: ") + ex.what());
+        return -2;
+    }
+}
+
+// ------------------------------------------------------------------ kernel timing
+// When enabled, every launch of the batched pipelines is bracketed by HIP events on
+// its own stream; categories: 0 panel, 1 swap+trsm, 2 gemm, 3 assembly, 4 tracelog,
+// 5 solve kernels, 6 other.  flops/bytes are the algorithmic counts of the launch.
+struct ProfRec {
+    int cat;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;
+
+template <class F>
+static void timed(int cat, double flops, double bytes, hipStream_t st, F&& launch) {
+    if (!g_prof_on) { launch(); return; }
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, st));
+    launch();
+    HIPCHK(hipEventRecord(b, st));
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof.push_back({cat, a, b, flops, bytes});
+}
+
+int profile_begin() {
+    return guarded([&]() -> int {
+        std::lock_guard<std::mutex> g(g_prof_mu);
+        for (auto& r : g_prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+        g_prof.clear();
+        g_prof_on = true;
+        return 0;
+    });
+}
+
+int profile_end(double* ms, double* flops, double* bytes, int* count, int ncat) {
+    return guarded([&]() -> int {
+        std::lock_guard<std::mutex> g(g_prof_mu);
+        g_prof_on = false;
+        for (int c = 0; c < ncat; c++) { ms[c] = 0; flops[c] = 0; bytes[c] = 0; count[c] = 0; }
+        for (auto& r : g_prof) {
+            HIPCHK(hipEventSynchronize(r.b));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+            if (r.cat < ncat) { ms[r.cat] += t; flops[r.cat] += r.flops; bytes[r.cat] += r.bytes; count[r.cat]++; }
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        g_prof.clear();
+        return 0;
+    });
+}
+
+MediumView view_of(const msgpu_medium* med) {
+    MediumView v;
+    v.d = med->d;
+    v.n = med->n;
+    v.np = padded(med->n);
+    v.m = scmodel_of(med);
+    v.pos = med->pos;
+    return v;
+}
+
+static constexpr int NB = 64;    // panel width == triangular block of the solves
+
+// Workspace budget for a batch of matrices (bytes), MSGPU_WORKSPACE_GB overrides.
+static size_t batch_budget() {
+    static size_t b = [] {
+        const char* e = getenv("MSGPU_WORKSPACE_GB");
+        double gb = e ? atof(e) : 16.0;
+        return (size_t)(gb * (1ull << 30));
+    }();
+    return b;
+}
+static int max_batch(int np, size_t elem, int mats_per_item) {
+    size_t per = (size_t)np * np * elem * mats_per_item;
+    size_t b = batch_budget() / per;
+    if (b < 1) b = 1;
+    if (b > 512) b = 512;
+    return (int)b;
+}
+
+// ------------------------------------------------------------------ LU core
+template <class T>
+static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, hipStream_t st) {
+    const long long ms = (long long)np * np;
+    HIPCHK(hipMemsetAsync(info, 0, sizeof(int) * nbatch, st));
+    const double es = sizeof(cx<T>);
+    for (int k0 = 0; k0 < np; k0 += NB) {
+        const int kend = k0 + NB, rest = np - kend;
+        timed(0, 8.0 * nbatch * ((double)(np - k0) * NB * NB / 2), es * nbatch * 2.0 * (np - k0) * NB, st,
+              [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st); });
+        timed(1, 8.0 * nbatch * (double)NB * NB / 2 * rest, es * nbatch * 2.0 * NB * rest, st,
+              [&] { launch_swap_trsm<T>(mats, np, ms, pmap, k0, NB, left ? 1 : 0, nbatch, st); });
+        if (kend < np)
+            timed(2, 8.0 * nbatch * (double)rest * rest * NB, es * nbatch * (2.0 * rest * rest + 2.0 * rest * NB), st,
+                  [&] {
+                      launch_gemm_sub<T>(mats + kend + (size_t)kend * np, mats + kend + (size_t)k0 * np,
+                                         mats + k0 + (size_t)kend * np, rest, rest, NB, np, np, np, ms, ms, ms, nbatch,
+                                         st);
+                  });
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// X <- M^-1 X for X (np x ncol, ld np) given the factor of factor() without left swaps.
+template <class T>
+static void lu_solve(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, const int* pmap, long long sx,
+                     hipStream_t st) {
+    const long long ms = (long long)np * np;
+    const int nq = np / NB;
+    for (int q = 0; q < nq; q++) {
+        const int k0 = q * NB;
+        launch_apply_map<T>(X, np, ncol, pmap, NB, q, sx, nbatch, st);
+        launch_trsm_block<T>(lu, X, np, ncol, k0, 0, ms, sx, nbatch, st);
+        if (k0 + NB < np)
+            launch_gemm_sub<T>(X + k0 + NB, lu + (k0 + NB) + (size_t)k0 * np, X + k0, np - k0 - NB, ncol, NB, np, np,
+                               np, sx, ms, sx, nbatch, st);
+    }
+    for (int q = nq - 1; q >= 0; q--) {
+        const int k0 = q * NB;
+        launch_trsm_block<T>(lu, X, np, ncol, k0, 1, ms, sx, nbatch, st);
+        if (k0 > 0)
+            launch_gemm_sub<T>(X, lu + (size_t)k0 * np, X + k0, k0, ncol, NB, np, np, np, sx, ms, sx, nbatch, st);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ batched trace-log
+template <class T>
+static void tracelog_impl(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, double* stats) {
+    Lease L;
+    const int np = mv.np;
+    DBuf<double> dpos((size_t)mv.n * mv.d);
+    DBuf<cx<double>> dks(nk), ddiag(nk), dout(nk);
+    DBuf<double> dstats(2 * (size_t)nk);
+    HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * mv.n * mv.d, hipMemcpyHostToDevice, L.st));
+    HIPCHK(hipMemcpyAsync(dks.p, ks, sizeof(cx<double>) * nk, hipMemcpyHostToDevice, L.st));
+    launch_diag(dks.p, nk, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
+    const int bmax = std::min(nk, max_batch(np, sizeof(cx<T>), 1));
+    DBuf<cx<T>> mats((size_t)bmax * np * np);
+    DBuf<int> ipiv((size_t)bmax * np), info(bmax), pmap((size_t)bmax * pmap_ints(np, NB));
+    for (int done = 0; done < nk; done += bmax) {
+        const int B = std::min(bmax, nk - done);
+        timed(3, 0.0, (double)B * np * np * sizeof(cx<T>), L.st, [&] {
+            launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag.p + done, nullptr, 0, mats.p,
+                               (long long)np * np, B, L.st);
+        });
+        factor<T>(mats.p, np, B, ipiv.p, info.p, pmap.p, false, L.st);
+        timed(4, 0.0, (double)B * np * sizeof(cx<T>), L.st, [&] {
+            launch_tracelog<T>(mats.p, mv.n, np, (long long)np * np, ipiv.p, dout.p + done, dstats.p + 2 * done, B,
+                               L.st);
+        });
+    }
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, L.st));
+    if (stats) HIPCHK(hipMemcpyAsync(stats, dstats.p, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, L.st));
+    L.sync();
+}
+
+int tracelog_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, double* stats,
+                    int precision) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    if (nk <= 0) return 0;
+    return guarded([&]() -> int {
+        if (precision == 1) tracelog_impl<float>(mv, nk, ks, out, stats);
+        else tracelog_impl<double>(mv, nk, ks, out, stats);
+        return 0;
+    });
+}
+
+// ------------------------------------------------------------------ batched d/dk log det
+template <class T>
+static void dk_impl(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out) {
+    Lease L;
+    const int np = mv.np;
+    DBuf<double> dpos((size_t)mv.n * mv.d);
+    DBuf<cx<double>> dks(nk), ddiag(nk), ddiag1(nk), dout(nk);
+    HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * mv.n * mv.d, hipMemcpyHostToDevice, L.st));
+    HIPCHK(hipMemcpyAsync(dks.p, ks, sizeof(cx<double>) * nk, hipMemcpyHostToDevice, L.st));
+    launch_diag(dks.p, nk, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
+    launch_diag(dks.p, nk, mv.m, mv.d, 1, ddiag1.p, nullptr, L.st);
+    const int bmax = std::min(nk, max_batch(np, sizeof(cx<T>), 2));
+    const long long ms = (long long)np * np;
+    DBuf<cx<T>> mats((size_t)bmax * ms), rhs((size_t)bmax * ms);
+    DBuf<int> ipiv((size_t)bmax * np), info(bmax), pmap((size_t)bmax * pmap_ints(np, NB));
+    for (int done = 0; done < nk; done += bmax) {
+        const int B = std::min(bmax, nk - done);
+        launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag.p + done, nullptr, 0, mats.p, ms, B, L.st);
+        launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag1.p + done, nullptr, 1, rhs.p, ms, B, L.st);
+        factor<T>(mats.p, np, B, ipiv.p, info.p, pmap.p, false, L.st);
+        lu_solve<T>(mats.p, rhs.p, np, np, B, pmap.p, ms, L.st);
+        launch_diag_sum<T>(rhs.p, mv.n, np, ms, dout.p + done, B, L.st);
+    }
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, L.st));
+    L.sync();
+}
+
+int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, int precision) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    if (nk <= 0) return 0;
+    return guarded([&]() -> int {
+        if (precision == 1) dk_impl<float>(mv, nk, ks, out);
+        else dk_impl<double>(mv, nk, ks, out);
+        return 0;
+    });
+}
+
+// ------------------------------------------------------------------ amplitude solve
+static double host_norm(const std::vector<cx<double>>& v, int n) {
+    double s = 0;
+    for (int i = 0; i < n; i++) s += v[i].x * v[i].x + v[i].y * v[i].y;
+    return std::sqrt(s);
+}
+
+int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>* a, int mixed, double tol) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    return guarded([&]() -> int {
+        Lease L;
+        const int np = mv.np, n = mv.n;
+        const long long ms = (long long)np * np;
+        DBuf<double> dpos((size_t)n * mv.d);
+        DBuf<cx<double>> dk(1), ddiag(1), phi(np), x(np), r(np);
+        DBuf<cx<double>> m64(ms);
+        DBuf<int> ipiv(np), info(1), pmap(pmap_ints(np, NB));
+        HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpyAsync(dk.p, &k, sizeof(k), hipMemcpyHostToDevice, L.st));
+        launch_diag(dk.p, 1, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
+        launch_incident(dpos.p, n, mv.d, np, k, source, 1.0, 0.0, phi.p, L.st);
+        launch_assemble<double>(dpos.p, n, mv.d, np, dk.p, ddiag.p, nullptr, 0, m64.p, ms, 1, L.st);
+        int steps = 0;
+        if (!mixed) {
+            HIPCHK(hipMemcpyAsync(x.p, phi.p, sizeof(cx<double>) * np, hipMemcpyDeviceToDevice, L.st));
+            factor<double>(m64.p, np, 1, ipiv.p, info.p, pmap.p, false, L.st);
+            lu_solve<double>(m64.p, x.p, np, 1, 1, pmap.p, np, L.st);
+        } else {
+            // complex64 LU, FP64 residuals: x <- x + M32^-1 (phi - M64 x)
+            if (tol <= 0) tol = 1e-12;
+            DBuf<cx<float>> m32(ms), d32(np);
+            launch_convert<double, float>(m64.p, m32.p, ms, L.st);
+            factor<float>(m32.p, np, 1, ipiv.p, info.p, pmap.p, false, L.st);
+            HIPCHK(hipMemsetAsync(x.p, 0, sizeof(cx<double>) * np, L.st));
+            std::vector<cx<double>> hphi(np), hr(np);
+            HIPCHK(hipMemcpyAsync(hphi.data(), phi.p, sizeof(cx<double>) * np, hipMemcpyDeviceToHost, L.st));
+            L.sync();
+            const double nphi = host_norm(hphi, n);
+            for (steps = 0; steps < 60; steps++) {
+                launch_residual(m64.p, x.p, phi.p, np, r.p, L.st);
+                HIPCHK(hipMemcpyAsync(hr.data(), r.p, sizeof(cx<double>) * np, hipMemcpyDeviceToHost, L.st));
+                L.sync();
+                if (host_norm(hr, n) <= tol * nphi) break;
+                launch_convert<double, float>(r.p, d32.p, np, L.st);
+                lu_solve<float>(m32.p, d32.p, np, 1, 1, pmap.p, np, L.st);
+                launch_axpy<float>(x.p, d32.p, np, L.st);
+            }
+        }
+        int hinfo = 0;
+        HIPCHK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, L.st));
+        HIPCHK(hipMemcpyAsync(a, x.p, sizeof(cx<double>) * n, hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        if (hinfo > 0)
+            printf("[LAPACK] Element U(%d,%d) is exactly zero, the solution could not be computed.\n", hinfo, hinfo);
+        return steps;
+    });
+}
+
+// ------------------------------------------------------------------ psi
+int wave_grid(const MediumView& mv, cx<double> k, int source, const cx<double>* a, int nx, int ny, double xmin,
+              double xmax, double ymin, double ymax, cx<double>* psi) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    return guarded([&]() -> int {
+        Lease L;
+        const int n = mv.n;
+        const size_t tot = (size_t)nx * ny;
+        DBuf<double> dpos((size_t)n * mv.d);
+        DBuf<cx<double>> da(n), dpsi(tot);
+        HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpyAsync(da.p, a, sizeof(cx<double>) * n, hipMemcpyHostToDevice, L.st));
+        const double dx = nx > 1 ? (xmax - xmin) / (nx - 1) : 0.0, dy = ny > 1 ? (ymax - ymin) / (ny - 1) : 0.0;
+        launch_wave(dpos.p, n, mv.d, da.p, k, source, nx, ny, xmin, dx, ymax, dy, nullptr, dpsi.p, L.st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(psi, dpsi.p, sizeof(cx<double>) * tot, hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        return 0;
+    });
+}
+
+int wave_points(const MediumView& mv, cx<double> k, int source, const cx<double>* a, int npt, const double* pts,
+                cx<double>* psi) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    return guarded([&]() -> int {
+        Lease L;
+        const int n = mv.n;
+        DBuf<double> dpos((size_t)n * mv.d), dpts((size_t)npt * mv.d);
+        DBuf<cx<double>> da(n), dpsi(npt);
+        HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpyAsync(dpts.p, pts, sizeof(double) * npt * mv.d, hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpyAsync(da.p, a, sizeof(cx<double>) * n, hipMemcpyHostToDevice, L.st));
+        launch_wave(dpos.p, n, mv.d, da.p, k, source, npt, 1, 0.0, 0.0, 0.0, 0.0, dpts.p, dpsi.p, L.st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(psi, dpsi.p, sizeof(cx<double>) * npt, hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        return 0;
+    });
+}
+
+// ------------------------------------------------------------------ host-buffer matrix services
+int build_matrix_host(const MediumView& mv, cx<double> k, int kind, cx<double>* out) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    return guarded([&]() -> int {
+        Lease L;
+        const int np = mv.np, n = mv.n;
+        DBuf<double> dpos((size_t)n * mv.d);
+        DBuf<cx<double>> dk(1), ddiag(1), diz(1), m((size_t)np * np);
+        HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpyAsync(dk.p, &k, sizeof(k), hipMemcpyHostToDevice, L.st));
+        launch_diag(dk.p, 1, mv.m, mv.d, kind, ddiag.p, diz.p, L.st);
+        launch_assemble<double>(dpos.p, n, mv.d, np, dk.p, ddiag.p, diz.p, kind, m.p, (long long)np * np, 1, L.st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy2DAsync(out, sizeof(cx<double>) * n, m.p, sizeof(cx<double>) * np, sizeof(cx<double>) * n, n,
+                                hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        return 0;
+    });
+}
+
+int incident_host(const MediumView& mv, cx<double> k, int source, cx<double>* out) {
+    return guarded([&]() -> int {
+        Lease L;
+        const int n = mv.n;
+        DBuf<double> dpos((size_t)n * mv.d);
+        DBuf<cx<double>> v(mv.np);
+        HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+        launch_incident(dpos.p, n, mv.d, mv.np, k, source, 1.0, 0.0, v.p, L.st);
+        HIPCHK(hipMemcpyAsync(out, v.p, sizeof(cx<double>) * n, hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        return 0;
+    });
+}
+
+int plane_wave_host(const MediumView& mv, cx<double> k, double theta_deg, cx<double>* out) {
+    return guarded([&]() -> int {
+        Lease L;
+        const int n = mv.n;
+        const double th = theta_deg * (3.1415926535897932385 / 180);
+        double ct = std::cos(th), stt = std::sin(th);
+        if (mv.d == 1) ct = ct > 0. ? 1. : -1.;
+        DBuf<double> dpos((size_t)n * mv.d);
+        DBuf<cx<double>> v(mv.np);
+        HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+        launch_incident(dpos.p, n, mv.d, mv.np, k, 2, ct, stt, v.p, L.st);
+        HIPCHK(hipMemcpyAsync(out, v.p, sizeof(cx<double>) * n, hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        return 0;
+    });
+}
+
+// Host matrix -> padded device matrix (identity padding).
+static void upload_padded(cx<double>* dm, int n, int np, const cx<double>* a, int lda, hipStream_t st) {
+    HIPCHK(hipMemsetAsync(dm, 0, sizeof(cx<double>) * (size_t)np * np, st));
+    HIPCHK(hipMemcpy2DAsync(dm, sizeof(cx<double>) * np, a, sizeof(cx<double>) * lda, sizeof(cx<double>) * n, n,
+                            hipMemcpyHostToDevice, st));
+    launch_pad_identity<double>(dm, n, np, 1, st);
+}
+
+int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, int ldb, cx<double>* tracelog,
+            int* info) {
+    return guarded([&]() -> int {
+        Lease L;
+        const int np = padded(n);
+        DBuf<cx<double>> m((size_t)np * np), x((size_t)np * (nrhs > 0 ? nrhs : 1)), tl(1);
+        DBuf<int> dpiv(np), dinfo(1), pmap(pmap_ints(np, NB));
+        upload_padded(m.p, n, np, a, lda, L.st);
+        if (nrhs > 0) {
+            HIPCHK(hipMemsetAsync(x.p, 0, sizeof(cx<double>) * (size_t)np * nrhs, L.st));
+            HIPCHK(hipMemcpy2DAsync(x.p, sizeof(cx<double>) * np, b, sizeof(cx<double>) * ldb, sizeof(cx<double>) * n,
+                                    nrhs, hipMemcpyHostToDevice, L.st));
+        }
+        // factor without left interchanges, solve, then apply them for the LAPACK-format output
+        factor<double>(m.p, np, 1, dpiv.p, dinfo.p, pmap.p, false, L.st);
+        if (tracelog) launch_tracelog<double>(m.p, n, np, (long long)np * np, dpiv.p, tl.p, nullptr, 1, L.st);
+        if (nrhs > 0) lu_solve<double>(m.p, x.p, np, nrhs, 1, pmap.p, np, L.st);
+        for (int k0 = NB; k0 < np; k0 += NB) launch_swap_left<double>(m.p, np, (long long)np * np, pmap.p, k0, NB, 1, L.st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy2DAsync(a, sizeof(cx<double>) * lda, m.p, sizeof(cx<double>) * np, sizeof(cx<double>) * n, n,
+                                hipMemcpyDeviceToHost, L.st));
+        HIPCHK(hipMemcpyAsync(ipiv, dpiv.p, sizeof(int) * n, hipMemcpyDeviceToHost, L.st));
+        HIPCHK(hipMemcpyAsync(info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, L.st));
+        if (nrhs > 0)
+            HIPCHK(hipMemcpy2DAsync(b, sizeof(cx<double>) * ldb, x.p, sizeof(cx<double>) * np, sizeof(cx<double>) * n,
+                                    nrhs, hipMemcpyDeviceToHost, L.st));
+        if (tracelog) HIPCHK(hipMemcpyAsync(tracelog, tl.p, sizeof(cx<double>), hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        return 0;
+    });
+}
+
+int lu_solve_host(int n, const cx<double>* lu, int lda, const int* ipiv, int nrhs, cx<double>* b, int ldb) {
+    return guarded([&]() -> int {
+        Lease L;
+        const int np = padded(n);
+        // LAPACK-format factor: apply ipiv to b sequentially on the host-side copy, then two triangular solves
+        std::vector<int> perm(np);
+        for (int i = 0; i < np; i++) perm[i] = i;
+        for (int i = 0; i < n; i++) std::swap(perm[i], perm[ipiv[i] - 1]);
+        DBuf<cx<double>> m((size_t)np * np), x((size_t)np * nrhs), y((size_t)np * nrhs);
+        DBuf<int> dperm(np);
+        upload_padded(m.p, n, np, lu, lda, L.st);
+        HIPCHK(hipMemsetAsync(x.p, 0, sizeof(cx<double>) * (size_t)np * nrhs, L.st));
+        HIPCHK(hipMemcpy2DAsync(x.p, sizeof(cx<double>) * np, b, sizeof(cx<double>) * ldb, sizeof(cx<double>) * n, nrhs,
+                                hipMemcpyHostToDevice, L.st));
+        HIPCHK(hipMemcpyAsync(dperm.p, perm.data(), sizeof(int) * np, hipMemcpyHostToDevice, L.st));
+        launch_gather_rows<double>(y.p, x.p, dperm.p, np, nrhs, 0, 0, 1, L.st);
+        const long long ms = (long long)np * np;
+        for (int k0 = 0; k0 < np; k0 += NB) {
+            launch_trsm_block<double>(m.p, y.p, np, nrhs, k0, 0, ms, 0, 1, L.st);
+            if (k0 + NB < np)
+                launch_gemm_sub<double>(y.p + k0 + NB, m.p + (k0 + NB) + (size_t)k0 * np, y.p + k0, np - k0 - NB, nrhs,
+                                        NB, np, np, np, 0, 0, 0, 1, L.st);
+        }
+        for (int k0 = np - NB; k0 >= 0; k0 -= NB) {
+            launch_trsm_block<double>(m.p, y.p, np, nrhs, k0, 1, ms, 0, 1, L.st);
+            if (k0 > 0)
+                launch_gemm_sub<double>(y.p, m.p + (size_t)k0 * np, y.p + k0, k0, nrhs, NB, np, np, np, 0, 0, 0, 1, L.st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy2DAsync(b, sizeof(cx<double>) * ldb, y.p, sizeof(cx<double>) * np, sizeof(cx<double>) * n, nrhs,
+                                hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        return 0;
+    });
+}
+
+}  // namespace msg

@@ -1,0 +1,207 @@
+// kern_assemble.hip -- device assembly of the multiple-scattering matrices, the
+// per-k diagonal, incident vectors and the psi grid.
+//
+// Layout: every matrix of a batch is Np x Np, column-major, interleaved complex
+// (re, im), leading dimension Np, matrices back to back (stride Np*Np).  Np is N
+// rounded up to a multiple of 64; the padding is the identity (diagonal 1,
+// zeros elsewhere), which leaves the determinant, the pivot sequence of the
+// first N columns and the solution of the first N unknowns unchanged.
+#include "kernels.h"
+
+namespace msg {
+
+// ------------------------------------------------------------------ diagonal
+// kind 0: F(k)^-1; kind 1: (F^-1(k+h) - F^-1(k-h))/(2h), h = 1e-4
+// (ms_matrix_util.c:68-81); kind 2: normalised matrix (diag i, izero = I_d(k,0)).
+__global__ void diag_kernel(const cx<double>* ks, int nk, ScModel m, int d, int kind, cx<double>* diag,
+                            cx<double>* izero) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nk) return;
+    cx<double> k = ks[l];
+    if (kind == 0) {
+        diag[l] = invf(m, d, k);
+    } else if (kind == 1) {
+        const cx<double> h{1e-4, 0.0};
+        diag[l] = (invf(m, d, k + h) - invf(m, d, k - h)) / (2.0 * h);
+    } else {
+        diag[l] = cx<double>{0.0, 1.0};
+        izero[l] = green_imag(d, k, 0.0);
+    }
+}
+
+void launch_diag(const cx<double>* ks, int nk, ScModel m, int d, int kind, cx<double>* diag, cx<double>* izero,
+                 hipStream_t st) {
+    hipLaunchKernelGGL(diag_kernel, dim3((nk + 63) / 64), dim3(64), 0, st, ks, nk, m, d, kind, diag, izero);
+}
+
+// ------------------------------------------------------------------ assembly
+// One workgroup per pair of 32x32 tiles (I <= J) of one matrix: it evaluates the
+// Green function once per unordered pair and writes both (I,J) and, through an
+// LDS transpose, (J,I), so M = M^T exactly (ms_matrix_util.c:36-41 mirrors the
+// upper triangle the same way).
+constexpr int AT = 32;
+
+template <class T>
+__global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict__ pos, int n, int d, int np,
+                                                       const cx<double>* __restrict__ ks,
+                                                       const cx<double>* __restrict__ diag,
+                                                       const cx<double>* __restrict__ izero, int kind,
+                                                       cx<T>* __restrict__ out, long long mstride) {
+    const int b = blockIdx.y;
+    const int nt = np / AT;
+    // decode the upper-triangular tile pair index
+    int t = blockIdx.x, I = 0;
+    while (t >= nt - I) { t -= nt - I; I++; }
+    const int J = I + t;
+    __shared__ double sx[2][AT][4];
+    __shared__ cx<T> tile[AT][AT + 1];
+    const int tid = threadIdx.x;
+    if (tid < 2 * AT) {
+        int side = tid / AT, a = tid % AT;
+        int atom = (side == 0 ? I : J) * AT + a;
+        for (int c = 0; c < 4; c++) sx[side][a][c] = (atom < n && c < d) ? pos[(size_t)atom * d + c] : 0.0;
+    }
+    __syncthreads();
+    const cx<double> kd = ks[b];
+    const cx<T> k{(T)kd.x, (T)kd.y};
+    const cx<double> dg = diag[b];
+    cx<double> iz{1.0, 0.0};
+    if (kind == 2) iz = izero[b];
+    cx<T>* M = out + (size_t)b * mstride;
+    const int ii = tid % AT, j0 = tid / AT;
+#pragma unroll
+    for (int q = 0; q < AT / 8; q++) {
+        const int jj = j0 + 8 * q;
+        const int gi = I * AT + ii, gj = J * AT + jj;
+        cx<T> v;
+        if (gi >= n || gj >= n) {
+            v = cx<T>{gi == gj ? T(1) : T(0), T(0)};
+        } else if (gi == gj) {
+            v = cx<T>{(T)dg.x, (T)dg.y};
+        } else {
+            double s = 0.0;
+            for (int c = 0; c < d; c++) {
+                double e = sx[0][ii][c] - sx[1][jj][c];
+                s += e * e;
+            }
+            T r = (T)sqrt(s);
+            if (kind == 1) {
+                v = neg_dk_green<T>(d, k, r);
+            } else if (kind == 2) {
+                const cx<T> ng = neg_green<T>(d, k, r);
+                const cx<double> g = cx<double>{(double)ng.x, (double)ng.y} / iz;
+                v = cx<T>{(T)g.x, (T)g.y};
+            } else {
+                v = neg_green<T>(d, k, r);
+            }
+        }
+        M[(size_t)gi + (size_t)gj * np] = v;
+        tile[jj][ii] = v;
+    }
+    if (I == J) return;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < AT / 8; q++) {
+        const int bb = j0 + 8 * q;        // column inside tile (J, I) is I*AT + bb
+        M[(size_t)(J * AT + ii) + (size_t)(I * AT + bb) * np] = tile[ii][bb];
+    }
+}
+
+template <class T>
+void launch_assemble(const double* pos, int n, int d, int np, const cx<double>* ks, const cx<double>* diag,
+                     const cx<double>* izero, int kind, cx<T>* out, long long mstride, int nbatch, hipStream_t st) {
+    const int nt = np / AT;
+    dim3 grid(nt * (nt + 1) / 2, nbatch);
+    hipLaunchKernelGGL(assemble_kernel<T>, grid, dim3(256), 0, st, pos, n, d, np, ks, diag, izero, kind, out,
+                       mstride);
+}
+template void launch_assemble<double>(const double*, int, int, int, const cx<double>*, const cx<double>*,
+                                      const cx<double>*, int, cx<double>*, long long, int, hipStream_t);
+template void launch_assemble<float>(const double*, int, int, int, const cx<double>*, const cx<double>*,
+                                     const cx<double>*, int, cx<float>*, long long, int, hipStream_t);
+
+// ------------------------------------------------------------------ incident vectors
+// source 0: plane wave e^{ik x_0} (incident_wave_util.c:76); 1: spherical G_d(k,|x|);
+// 2: build_plane_wave with angle theta (ms_matrix_util.c:88-106).  Padding rows get 0.
+__global__ void incident_kernel(const double* __restrict__ pos, int n, int d, int np, cx<double> k, int source,
+                                double ct, double stt, cx<double>* __restrict__ phi) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    if (i >= n) { phi[i] = cx<double>{0.0, 0.0}; return; }
+    const double* x = pos + (size_t)i * d;
+    cx<double> v;
+    if (source == 0) {
+        v = cexp(cx<double>{-k.y * x[0], k.x * x[0]});
+    } else if (source == 1) {
+        double s = 0.0;
+        for (int c = 0; c < d; c++) s += x[c] * x[c];
+        v = green<double>(d, k, sqrt(s));
+    } else {
+        double proj = (d == 1) ? ct * x[0] : ct * x[0] + stt * x[1];
+        v = cexp(cx<double>{-k.y * proj, k.x * proj});
+    }
+    phi[i] = v;
+}
+
+void launch_incident(const double* pos, int n, int d, int np, cx<double> k, int source, double ct, double st_,
+                     cx<double>* phi, hipStream_t st) {
+    hipLaunchKernelGGL(incident_kernel, dim3((np + 255) / 256), dim3(256), 0, st, pos, n, d, np, k, source, ct, st_,
+                       phi);
+}
+
+// ------------------------------------------------------------------ psi grid
+// psi(r) = phi(r) + sum_i a_i G_d(k, |x_i - r|) at r = (xmin + ix dx, ymax - iy dy, 0, ...)
+// (incident_wave_util.c:87-96, wave_function_util.c:76-81).  One thread per
+// grid point; scatterers and amplitudes are staged through LDS in chunks.
+constexpr int WCH = 256;
+
+__global__ __launch_bounds__(256) void wave_kernel(const double* __restrict__ pos, int n, int d,
+                                                   const cx<double>* __restrict__ a, cx<double> k, int source,
+                                                   int nx, int ny, double xmin, double dx, double ymax, double dy,
+                                                   const double* __restrict__ pts, cx<double>* __restrict__ psi) {
+    __shared__ double sp[WCH][4];
+    __shared__ cx<double> sa[WCH];
+    const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = pts ? (long long)nx : (long long)nx * ny;
+    const bool live = l < tot;
+    const int ix = live ? (int)(l % nx) : 0, iy = live ? (int)(l / nx) : 0;
+    double pt[4] = {xmin + ix * dx, (d > 1) ? ymax - iy * dy : 0.0, 0.0, 0.0};
+    if (pts && live)
+        for (int c = 0; c < d; c++) pt[c] = pts[(size_t)l * d + c];
+    cx<double> acc{0.0, 0.0};
+    for (int base = 0; base < n; base += WCH) {
+        __syncthreads();
+        const int cnt = min(WCH, n - base);
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+            for (int c = 0; c < 4; c++) sp[q][c] = (c < d) ? pos[(size_t)(base + q) * d + c] : 0.0;
+            sa[q] = a[base + q];
+        }
+        __syncthreads();
+        if (live) {
+            for (int q = 0; q < cnt; q++) {
+                double s = 0.0;
+                for (int c = 0; c < d; c++) { double e = sp[q][c] - pt[c]; s += e * e; }
+                acc = acc + sa[q] * green<double>(d, k, sqrt(s));
+            }
+        }
+    }
+    if (!live) return;
+    cx<double> inc;
+    if (source == 0) {
+        inc = cexp(cx<double>{-k.y * pt[0], k.x * pt[0]});
+    } else {
+        double s = 0.0;
+        for (int c = 0; c < d; c++) s += pt[c] * pt[c];
+        inc = green<double>(d, k, sqrt(s));
+    }
+    psi[l] = acc + inc;
+}
+
+void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int nx, int ny,
+                 double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi, hipStream_t st) {
+    long long tot = pts ? (long long)nx : (long long)nx * ny;
+    hipLaunchKernelGGL(wave_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, pos, n, d, a, k, source, nx,
+                       ny, xmin, dx, ymax, dy, pts, psi);
+}
+
+}  // namespace msg

@@ -1,0 +1,819 @@
+// kern_lu.hip -- batched right-looking blocked LU with partial pivoting for
+// complex matrices (complex128 or complex64), and the kernels that consume it.
+//
+// Per block step k0 (panel width nb):
+//   lu_panel    one workgroup per matrix factors the Np x nb panel.  Rows are
+//               spread over 256 threads (row g belongs to thread (g-c0)%256) and
+//               held in registers, S columns at a time; pivot search is a wave
+//               shuffle argmax followed by a 4-way LDS vote (one barrier per
+//               column).  Pivot rule = LAPACK izamax: largest |Re|+|Im|, first
+//               index on ties.  The panel's interchanges are also folded into a
+//               composite row map (<= 2nb rows) that the next kernel applies as
+//               one gather.
+//   swap_trsm   one workgroup per 64-column tile: applies the row map, and for
+//               tiles right of the panel solves U12 = L11^-1 A12 in registers
+//               (4 lanes per column, wave shuffles, no barriers).
+//   gemm_sub    C -= A*B on MFMA: the complex product is the real product of the
+//               interleaved C (2m x n) with [[Ar,-Ai],[Ai,Ar]] (2m x 2k) and the
+//               interleaved B (2k x n), so A and B are read in their natural
+//               column-major complex layout; v_mfma_{f64,f32}_16x16x4.
+#include "kernels.h"
+
+namespace msg {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <class T> struct mfma;
+// Operands (both types): lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Result: lane l, register r holds D[row(l,r)][l%16], where the row mapping differs
+// between the f64 and the f32 instruction (checked on gfx950 with an in-bounds probe).
+template <> struct mfma<double> {
+    using acc = d4v;
+    static __device__ __forceinline__ acc op(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct mfma<float> {
+    using acc = f4v;
+    static __device__ __forceinline__ acc op(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+// ------------------------------------------------------------------ panel
+// pmap layout per matrix and panel q: [cnt, dst[2*nb], src[2*nb]] (1 + 4*nb ints).
+template <class T, int S, int R>
+__global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, int np, long long mstride,
+                                                       int* __restrict__ ipiv_all, int* __restrict__ info_all,
+                                                       int* __restrict__ pmap_all, int k0, int nb) {
+    extern __shared__ int s_slot[];                 // np ints: row -> slot of the composite map
+    __shared__ T s_abs[2][4];
+    __shared__ int s_idx[2][4];
+    __shared__ cx<T> s_row[2][4][S];
+    __shared__ cx<T> s_rowj[2][S];
+    __shared__ cx<T> s_L[S][S];
+    __shared__ cx<T> s_U[S][64];
+    __shared__ int s_dst[128], s_src[128], s_cnt;
+    __shared__ int s_sd[2 * S], s_ss[2 * S], s_scnt;
+
+    const int b = blockIdx.x;
+    cx<T>* M = A + (size_t)b * mstride;
+    int* ipiv = ipiv_all + (size_t)b * np;
+    const int nq = np / nb, q = k0 / nb;
+    int* pmap = pmap_all + ((size_t)b * nq + q) * (1 + 4 * nb);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int kend = min(k0 + nb, np);
+
+    for (int i = tid; i < np; i += 256) s_slot[i] = -1;
+    if (tid == 0) s_cnt = 0;
+    __syncthreads();
+
+    for (int c0 = k0; c0 < kend; c0 += S) {
+        cx<T> x[R][S];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int g = c0 + tid + 256 * r;
+#pragma unroll
+            for (int c = 0; c < S; c++)
+                x[r][c] = (g < np) ? M[(size_t)g + (size_t)(c0 + c) * np] : cx<T>{T(0), T(0)};
+        }
+#pragma unroll
+        for (int jj = 0; jj < S; jj++) {
+            const int j = c0 + jj, buf = jj & 1;
+            T best = T(-1);
+            int bidx = 0x7fffffff, br = -1;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int g = c0 + tid + 256 * r;
+                if (g >= j && g < np) {
+                    T v = abs1(x[r][jj]);
+                    if (v > best) { best = v; bidx = g; br = r; }
+                }
+            }
+            const int mine = bidx;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                T ov = __shfl_xor(best, off, 64);
+                int oi = __shfl_xor(bidx, off, 64);
+                if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+            }
+            if (lane == 0) { s_abs[buf][wv] = best; s_idx[buf][wv] = bidx; }
+            if (br >= 0 && mine == bidx) {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if (r == br)
+#pragma unroll
+                        for (int c = 0; c < S; c++) s_row[buf][wv][c] = x[r][c];
+            }
+            if (tid == jj) {        // row j = c0 + jj lives in thread jj, register row 0
+#pragma unroll
+                for (int c = 0; c < S; c++) s_rowj[buf][c] = x[0][c];
+            }
+            __syncthreads();
+            int bw = 0;
+#pragma unroll
+            for (int w = 1; w < 4; w++)
+                if (s_abs[buf][w] > s_abs[buf][bw] || (s_abs[buf][w] == s_abs[buf][bw] && s_idx[buf][w] < s_idx[buf][bw]))
+                    bw = w;
+            const int p = s_idx[buf][bw];
+            cx<T> piv[S];
+#pragma unroll
+            for (int c = 0; c < S; c++) piv[c] = s_row[buf][bw][c];
+            const cx<T> pv = piv[jj];
+            const bool zero = (pv.x == T(0) && pv.y == T(0));
+            if (tid == 0) {
+                ipiv[j] = p + 1;
+                if (zero && info_all[b] == 0) info_all[b] = j + 1;
+                // composite map of the panel's interchanges
+                int sj = s_slot[j];
+                if (sj < 0) { sj = s_cnt++; s_slot[j] = sj; s_dst[sj] = j; s_src[sj] = j; }
+                int sp = s_slot[p];
+                if (sp < 0) { sp = s_cnt++; s_slot[p] = sp; s_dst[sp] = p; s_src[sp] = p; }
+                int t = s_src[sj]; s_src[sj] = s_src[sp]; s_src[sp] = t;
+            }
+            if (p != j) {
+                const int op = (p - c0) & 255, rp = (p - c0) >> 8;
+                if (tid == op) {
+#pragma unroll
+                    for (int r = 0; r < R; r++)
+                        if (r == rp)
+#pragma unroll
+                            for (int c = 0; c < S; c++) x[r][c] = s_rowj[buf][c];
+                }
+                if (tid == jj) {
+#pragma unroll
+                    for (int c = 0; c < S; c++) x[0][c] = piv[c];
+                }
+            }
+            const cx<T> rinv = zero ? cx<T>{T(1), T(0)} : inv(pv);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int g = c0 + tid + 256 * r;
+                if (g > j && g < np) {
+                    const cx<T> l = x[r][jj] * rinv;
+                    x[r][jj] = l;
+#pragma unroll
+                    for (int c = jj + 1; c < S; c++) x[r][c] = x[r][c] - l * piv[c];
+                }
+            }
+        }
+        // write the factored sub-panel back
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int g = c0 + tid + 256 * r;
+            if (g < np)
+#pragma unroll
+                for (int c = 0; c < S; c++) M[(size_t)g + (size_t)(c0 + c) * np] = x[r][c];
+        }
+        if (tid < S) {
+#pragma unroll
+            for (int c = 0; c < S; c++) s_L[tid][c] = x[0][c];
+        }
+        __syncthreads();
+        const int nleft = c0 - k0, nright = kend - c0 - S;
+        if (nleft + nright > 0) {
+            // composite map of this sub-panel's S interchanges
+            if (tid == 0) {
+                int cnt = 0;
+                for (int jj = 0; jj < S; jj++) {
+                    const int j = c0 + jj, p = ipiv[j] - 1;
+                    int sj = -1, sp = -1;
+                    for (int e = 0; e < cnt; e++) { if (s_sd[e] == j) sj = e; if (s_sd[e] == p) sp = e; }
+                    if (sj < 0) { sj = cnt++; s_sd[sj] = j; s_ss[sj] = j; }
+                    if (sp < 0) { sp = cnt++; s_sd[sp] = p; s_ss[sp] = p; }
+                    int t = s_ss[sj]; s_ss[sj] = s_ss[sp]; s_ss[sp] = t;
+                }
+                s_scnt = cnt;
+            }
+            __syncthreads();
+            // apply it to the panel columns outside the sub-panel: gather all, then scatter
+            const int cnt = s_scnt, ncol = nleft + nright, pairs = cnt * ncol;
+            cx<T> tmp[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int idx = tid + 256 * u;
+                if (idx < pairs) {
+                    const int e = idx % cnt, ci = idx / cnt;
+                    const int col = ci < nleft ? k0 + ci : c0 + S + (ci - nleft);
+                    tmp[u] = M[(size_t)s_ss[e] + (size_t)col * np];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int idx = tid + 256 * u;
+                if (idx < pairs) {
+                    const int e = idx % cnt, ci = idx / cnt;
+                    const int col = ci < nleft ? k0 + ci : c0 + S + (ci - nleft);
+                    M[(size_t)s_sd[e] + (size_t)col * np] = tmp[u];
+                }
+            }
+            __syncthreads();
+        }
+        if (nright > 0) {
+            // U12' = L11'^-1 A12' for the panel columns right of the sub-panel
+            if (tid < nright) {
+                const int col = c0 + S + tid;
+                cx<T> u[S];
+#pragma unroll
+                for (int i = 0; i < S; i++) u[i] = M[(size_t)(c0 + i) + (size_t)col * np];
+#pragma unroll
+                for (int i = 1; i < S; i++)
+#pragma unroll
+                    for (int m = 0; m < i; m++) u[i] = u[i] - s_L[i][m] * u[m];
+#pragma unroll
+                for (int i = 0; i < S; i++) {
+                    M[(size_t)(c0 + i) + (size_t)col * np] = u[i];
+                    s_U[i][tid] = u[i];
+                }
+            }
+            __syncthreads();
+            // rank-S update of rows below the sub-panel: WR columns x RG register rows per round
+            constexpr int WR = sizeof(T) == 8 ? 4 : 8;
+            constexpr int RG = (R * WR * (int)sizeof(T) <= 128) ? R : (128 / (WR * (int)sizeof(T)));
+            for (int w0 = 0; w0 < nright; w0 += WR) {
+#pragma unroll
+                for (int r0 = 0; r0 < R; r0 += RG) {
+                    cx<T> av[RG][WR];
+#pragma unroll
+                    for (int rr = 0; rr < RG; rr++) {
+                        const int g = c0 + tid + 256 * (r0 + rr);
+#pragma unroll
+                        for (int w = 0; w < WR; w++)
+                            if (g >= c0 + S && g < np && w0 + w < nright)
+                                av[rr][w] = M[(size_t)g + (size_t)(c0 + S + w0 + w) * np];
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < RG; rr++) {
+                        const int g = c0 + tid + 256 * (r0 + rr);
+#pragma unroll
+                        for (int w = 0; w < WR; w++) {
+                            if (g >= c0 + S && g < np && w0 + w < nright) {
+                                cx<T> a = av[rr][w];
+#pragma unroll
+                                for (int m = 0; m < S; m++) a = a - x[r0 + rr][m] * s_U[m][w0 + w];
+                                M[(size_t)g + (size_t)(c0 + S + w0 + w) * np] = a;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // publish the panel's composite row map
+    __syncthreads();
+    const int cnt = s_cnt;
+    if (tid == 0) pmap[0] = cnt;
+    for (int e = tid; e < cnt; e += 256) {
+        pmap[1 + e] = s_dst[e];
+        pmap[1 + 2 * nb + e] = s_src[e];
+    }
+}
+
+// ------------------------------------------------------------------ in-register triangular solves
+// Four lanes per right-hand side: lane qq of a group holds rows 4m+qq in xv[m].
+// The register file is rotated by one after every 4 rows, so the row being
+// eliminated is always xv[0] (forward) or xv[15] (backward) and every register
+// index is a compile-time constant while the 16-step loop stays rolled.
+template <class T>
+__device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], const cx<T> (*sL)[65], int nb, int qq, int lane) {
+#pragma unroll 1
+    for (int ib = 0; ib < 16; ib++) {
+#pragma unroll
+        for (int iq = 0; iq < 4; iq++) {
+            const int i = 4 * ib + iq;
+            const int src = (lane & ~3) | iq;
+            const cx<T> xi{__shfl(xv[0].x, src, 64), __shfl(xv[0].y, src, 64)};
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const int row = 4 * (m + ib) + qq;
+                if (m + ib < 16 && row > i && row < nb) xv[m] = xv[m] - sL[row][i] * xi;
+            }
+        }
+        const cx<T> t = xv[0];
+#pragma unroll
+        for (int m = 0; m < 15; m++) xv[m] = xv[m + 1];
+        xv[15] = t;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], const cx<T> (*sL)[65], int qq, int lane) {
+#pragma unroll 1
+    for (int ib = 15; ib >= 0; ib--) {
+#pragma unroll
+        for (int iq = 3; iq >= 0; iq--) {
+            const int i = 4 * ib + iq;
+            if (qq == iq) xv[15] = xv[15] / sL[i][i];
+            const int src = (lane & ~3) | iq;
+            const cx<T> xi{__shfl(xv[15].x, src, 64), __shfl(xv[15].y, src, 64)};
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const int row = 4 * (m - (15 - ib)) + qq;
+                if (m >= 15 - ib && row < i) xv[m] = xv[m] - sL[row][i] * xi;
+            }
+        }
+        const cx<T> t = xv[15];
+#pragma unroll
+        for (int m = 15; m > 0; m--) xv[m] = xv[m - 1];
+        xv[0] = t;
+    }
+}
+
+// ------------------------------------------------------------------ swap + TRSM
+// grid.x = column tiles: the first nleft_t tiles cover [0, k0) (LAPACK-format L),
+// the rest cover [kend, ncols).  Right tiles also get U12 = L11^-1 A12.
+template <class T>
+__global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, int np, long long mstride,
+                                                        const int* __restrict__ pmap_all, int k0, int nb,
+                                                        int nleft_t, int do_trsm) {
+    __shared__ int s_dst[128], s_src[128];
+    __shared__ cx<T> sL[64][65];
+    const int b = blockIdx.y;
+    cx<T>* M = A + (size_t)b * mstride;
+    const int nq = np / nb, q = k0 / nb;
+    const int* pmap = pmap_all + ((size_t)b * nq + q) * (1 + 4 * nb);
+    const int kend = min(k0 + nb, np);
+    const int tid = threadIdx.x;
+    const bool left = (int)blockIdx.x < nleft_t;
+    const int col0 = left ? blockIdx.x * 64 : kend + (blockIdx.x - nleft_t) * 64;
+    const int colend = left ? min(col0 + 64, k0) : min(col0 + 64, np);
+    const int ncol = colend - col0;
+    const int cnt = pmap[0];
+    for (int e = tid; e < cnt; e += 256) { s_dst[e] = pmap[1 + e]; s_src[e] = pmap[1 + 2 * nb + e]; }
+    if (!left && do_trsm) {
+        // unit-lower L11 (nb x nb) of the panel
+        for (int e = tid; e < nb * nb; e += 256) {
+            const int i = e % nb, m = e / nb;
+            sL[i][m] = M[(size_t)(k0 + i) + (size_t)(k0 + m) * np];
+        }
+    }
+    __syncthreads();
+    // apply the row map, 8 columns per round (all reads of a column precede its writes)
+    for (int cb = 0; cb < ncol; cb += 8) {
+        const int pairs = cnt * min(8, ncol - cb);
+        cx<T> tmp[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int idx = tid + 256 * u;
+            if (idx < pairs) {
+                const int e = idx % cnt, c = col0 + cb + idx / cnt;
+                tmp[u] = M[(size_t)s_src[e] + (size_t)c * np];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int idx = tid + 256 * u;
+            if (idx < pairs) {
+                const int e = idx % cnt, c = col0 + cb + idx / cnt;
+                M[(size_t)s_dst[e] + (size_t)c * np] = tmp[u];
+            }
+        }
+        __syncthreads();
+    }
+    if (left || !do_trsm) return;
+    // TRSM: 4 lanes per column; lane q of a group holds rows i = 4m + q, m < nb/4
+    const int cl = tid >> 2, qq = tid & 3, lane = tid & 63;
+    const int col = col0 + cl;
+    const bool live = cl < ncol;
+    constexpr int MR = 16;               // nb <= 64
+    cx<T> xv[MR];
+#pragma unroll
+    for (int m = 0; m < MR; m++) {
+        const int i = 4 * m + qq;
+        xv[m] = (live && i < nb) ? M[(size_t)(k0 + i) + (size_t)col * np] : cx<T>{T(0), T(0)};
+    }
+    lower_unit_solve<T>(xv, sL, nb, qq, lane);
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < MR; m++) {
+            const int i = 4 * m + qq;
+            if (i < nb) M[(size_t)(k0 + i) + (size_t)col * np] = xv[m];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ GEMM: C -= A B
+constexpr int GT = 64;     // complex tile edge (rows and columns)
+constexpr int KC = 16;     // complex K chunk staged in LDS
+constexpr int PADA = 2, PADB = 2;
+
+template <class T>
+__global__ __launch_bounds__(256) void gemm_sub_kernel(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n,
+                                                       int kk, int lda, int ldb, int ldc, long long sC,
+                                                       long long sA, long long sB, int tiles_m, int tiles_n,
+                                                       int nbatch) {
+    __shared__ T lA[KC][2 * GT + PADA];
+    __shared__ T lB[GT][2 * KC + PADB];
+    const int tiles = tiles_m * tiles_n;
+    int g = blockIdx.x, b, t;
+    if ((nbatch & 7) == 0) {   // keep every tile of a matrix on one XCD (dispatch is round-robin over 8)
+        const int xcd = g & 7, j = g >> 3;
+        b = xcd + 8 * (j / tiles);
+        t = j % tiles;
+    } else {
+        b = g / tiles;
+        t = g % tiles;
+    }
+    const int row0 = (t % tiles_m) * GT, col0 = (t / tiles_m) * GT;
+    C += (size_t)b * sC; A += (size_t)b * sA; B += (size_t)b * sB;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w & 1, wc = w >> 1;
+    using acc_t = typename mfma<T>::acc;
+    acc_t acc[4][2];
+    // load C into the MFMA output layout (one real component per register)
+    T* Ct = reinterpret_cast<T*>(C);
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int ccol = col0 + wc * 32 + cb * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = wr * 64 + rb * 16 + mfma<T>::row(lane, r);
+                const int crow = row0 + (rr >> 1);
+                acc[rb][cb][r] = (ccol < n && crow < m) ? Ct[2 * ((size_t)crow + (size_t)ccol * ldc) + (rr & 1)] : T(0);
+            }
+        }
+    for (int kc0 = 0; kc0 < kk; kc0 += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < KC * GT / 256; u++) {
+            const int e = tid + 256 * u;
+            const int kcol = e / GT, row = e % GT;
+            cx<T> v{T(0), T(0)};
+            if (row0 + row < m && kc0 + kcol < kk) v = A[(size_t)(row0 + row) + (size_t)(kc0 + kcol) * lda];
+            lA[kcol][2 * row] = v.x;
+            lA[kcol][2 * row + 1] = v.y;
+        }
+#pragma unroll
+        for (int u = 0; u < KC * GT / 256; u++) {
+            const int e = tid + 256 * u;
+            const int col = e / KC, kr = e % KC;
+            cx<T> v{T(0), T(0)};
+            if (col0 + col < n && kc0 + kr < kk) v = B[(size_t)(kc0 + kr) + (size_t)(col0 + col) * ldb];
+            lB[col][2 * kr] = v.x;
+            lB[col][2 * kr + 1] = v.y;
+        }
+        __syncthreads();
+        const int kq = lane >> 4, rot = kq & 1;
+#pragma unroll
+        for (int ks = 0; ks < 2 * KC; ks += 4) {
+            const int p = (ks >> 1) + (kq >> 1);
+            T av[4], bv[2];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const int rr = wr * 64 + rb * 16 + (lane & 15);
+                const int comp = rr & 1;
+                T v;
+                if (!rot) v = -lA[p][rr];
+                else v = comp ? -lA[p][rr - 1] : lA[p][rr + 1];
+                av[rb] = v;
+            }
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) bv[cb] = lB[wc * 32 + cb * 16 + (lane & 15)][ks + kq];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++) acc[rb][cb] = mfma<T>::op(av[rb], bv[cb], acc[rb][cb]);
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int ccol = col0 + wc * 32 + cb * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = wr * 64 + rb * 16 + mfma<T>::row(lane, r);
+                const int crow = row0 + (rr >> 1);
+                if (ccol < n && crow < m) Ct[2 * ((size_t)crow + (size_t)ccol * ldc) + (rr & 1)] = acc[rb][cb][r];
+            }
+        }
+}
+
+// ------------------------------------------------------------------ tracelog
+// out[b] = (sum_i log U_ii + i pi [odd interchanges]) / n  (ms_matrix_util.c:185-202),
+// accumulated in FP64; stats[b] = {max_i log|U_ii|, min_i log|U_ii|}.
+template <class T>
+__global__ __launch_bounds__(256) void tracelog_kernel(const cx<T>* __restrict__ A, int n, int np,
+                                                       long long mstride, const int* __restrict__ ipiv_all,
+                                                       cx<double>* __restrict__ out, double* __restrict__ stats) {
+    __shared__ double sre[256], sim[256], smax[256], smin[256];
+    __shared__ int sodd[256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const cx<T>* M = A + (size_t)b * mstride;
+    const int* ipiv = ipiv_all + (size_t)b * np;
+    double re = 0.0, im = 0.0, mx = -INFINITY, mn = INFINITY;
+    int odd = 0;
+    for (int i = tid; i < n; i += 256) {
+        const cx<T> u = M[(size_t)i + (size_t)i * np];
+        const double la = log(hypot((double)u.x, (double)u.y));
+        re += la;
+        im += atan2((double)u.y, (double)u.x);
+        mx = fmax(mx, la);
+        mn = fmin(mn, la);
+        odd ^= (ipiv[i] != i + 1);
+    }
+    sre[tid] = re; sim[tid] = im; smax[tid] = mx; smin[tid] = mn; sodd[tid] = odd;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            sre[tid] += sre[tid + s];
+            sim[tid] += sim[tid + s];
+            smax[tid] = fmax(smax[tid], smax[tid + s]);
+            smin[tid] = fmin(smin[tid], smin[tid + s]);
+            sodd[tid] ^= sodd[tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double imt = sim[0] + (sodd[0] ? kPi : 0.0);
+        out[b] = cx<double>{sre[0] / n, imt / n};
+        if (stats) { stats[2 * b] = smax[0]; stats[2 * b + 1] = smin[0]; }
+    }
+}
+
+// ------------------------------------------------------------------ permutations
+// perm[i] = source row of row i of P*B, from the panels' composite maps.
+__global__ __launch_bounds__(256) void build_perm_kernel(const int* __restrict__ pmap_all, int np, int nb,
+                                                         int* __restrict__ perm_all) {
+    extern __shared__ int sp[];
+    const int b = blockIdx.x, tid = threadIdx.x, nq = np / nb;
+    for (int i = tid; i < np; i += 256) sp[i] = i;
+    __syncthreads();
+    for (int q = 0; q < nq; q++) {
+        const int* pm = pmap_all + ((size_t)b * nq + q) * (1 + 4 * nb);
+        const int cnt = pm[0];
+        int v = 0;
+        if (tid < cnt) v = sp[pm[1 + 2 * nb + tid]];
+        __syncthreads();
+        if (tid < cnt) sp[pm[1 + tid]] = v;
+        __syncthreads();
+    }
+    for (int i = tid; i < np; i += 256) perm_all[(size_t)b * np + i] = sp[i];
+}
+
+// dst[b][i, c] = src[b][perm[b][i], c] for i < np, c < ncol (column-major, ld np).
+template <class T>
+__global__ void gather_rows_kernel(cx<T>* __restrict__ dst, const cx<T>* __restrict__ src,
+                                   const int* __restrict__ perm_all, int np, int ncol, long long sdst,
+                                   long long ssrc) {
+    const int b = blockIdx.z;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)np * ncol) return;
+    const int i = (int)(e % np), c = (int)(e / np);
+    const int* perm = perm_all + (size_t)b * np;
+    dst[(size_t)b * sdst + (size_t)i + (size_t)c * np] = src[(size_t)b * ssrc + (size_t)perm[i] + (size_t)c * np];
+}
+
+// ------------------------------------------------------------------ triangular block solve
+// Rows [j0, j0+64) of B (np x ncol, ld np) against the 64x64 diagonal block of the
+// factor: UPPER=0 unit-lower forward, UPPER=1 non-unit upper backward.
+template <class T, int UPPER>
+__global__ __launch_bounds__(256) void trsm_block_kernel(const cx<T>* __restrict__ LU, cx<T>* __restrict__ B,
+                                                         int np, int ncol, int j0, long long slu, long long sb) {
+    __shared__ cx<T> sL[64][65];
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const cx<T>* F = LU + (size_t)b * slu;
+    cx<T>* X = B + (size_t)b * sb;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int i = e % 64, m = e / 64;
+        sL[i][m] = F[(size_t)(j0 + i) + (size_t)(j0 + m) * np];
+    }
+    __syncthreads();
+    const int cl = tid >> 2, qq = tid & 3;
+    const int col = blockIdx.x * 64 + cl;
+    const bool live = col < ncol;
+    cx<T> xv[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const int i = 4 * m + qq;
+        xv[m] = live ? X[(size_t)(j0 + i) + (size_t)col * np] : cx<T>{T(0), T(0)};
+    }
+    if (!UPPER) lower_unit_solve<T>(xv, sL, 64, qq, lane);
+    else upper_solve<T>(xv, sL, qq, lane);
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) X[(size_t)(j0 + 4 * m + qq) + (size_t)col * np] = xv[m];
+    }
+}
+
+// out[b] = sum_{i<n} X[b][i,i] in FP64.
+template <class T>
+__global__ __launch_bounds__(256) void diag_sum_kernel(const cx<T>* __restrict__ X, int n, int ld, long long sx,
+                                                       cx<double>* __restrict__ out) {
+    __shared__ double sre[256], sim[256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    double re = 0.0, im = 0.0;
+    for (int i = tid; i < n; i += 256) {
+        const cx<T> v = X[(size_t)b * sx + (size_t)i + (size_t)i * ld];
+        re += (double)v.x;
+        im += (double)v.y;
+    }
+    sre[tid] = re; sim[tid] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) { sre[tid] += sre[tid + s]; sim[tid] += sim[tid + s]; }
+        __syncthreads();
+    }
+    if (tid == 0) out[b] = cx<double>{sre[0], sim[0]};
+}
+
+// y (T2) = x (T1) elementwise, count complex entries.
+template <class T1, class T2>
+__global__ void convert_kernel(const cx<T1>* __restrict__ x, cx<T2>* __restrict__ y, long long count) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < count) y[e] = cx<T2>{(T2)x[e].x, (T2)x[e].y};
+}
+
+// r = phi - M a for one matrix (np x np, FP64), one thread per row.
+__global__ void residual_kernel(const cx<double>* __restrict__ M, const cx<double>* __restrict__ a,
+                                const cx<double>* __restrict__ phi, int np, cx<double>* __restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    cx<double> s = phi[i];
+    for (int j = 0; j < np; j++) s = s - M[(size_t)i + (size_t)j * np] * a[j];
+    r[i] = s;
+}
+
+// a += d (FP64 += FP32-or-FP64 correction)
+template <class T>
+__global__ void axpy_kernel(cx<double>* __restrict__ a, const cx<T>* __restrict__ d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = a[i] + cx<double>{(double)d[i].x, (double)d[i].y};
+}
+
+
+// Apply panel q's composite row map to X (np x ncol, ld np): 8 columns per workgroup.
+template <class T>
+__global__ __launch_bounds__(256) void apply_map_kernel(cx<T>* __restrict__ X, int np, int ncol,
+                                                        const int* __restrict__ pmap_all, int nb, int q,
+                                                        long long sx) {
+    __shared__ int s_dst[128], s_src[128];
+    const int b = blockIdx.y, tid = threadIdx.x, nq = np / nb;
+    const int* pm = pmap_all + ((size_t)b * nq + q) * (1 + 4 * nb);
+    cx<T>* M = X + (size_t)b * sx;
+    const int cnt = pm[0];
+    for (int e = tid; e < cnt; e += 256) { s_dst[e] = pm[1 + e]; s_src[e] = pm[1 + 2 * nb + e]; }
+    __syncthreads();
+    const int c0 = blockIdx.x * 8, nc = min(8, ncol - c0), pairs = cnt * nc;
+    cx<T> tmp[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int idx = tid + 256 * u;
+        if (idx < pairs) tmp[u] = M[(size_t)s_src[idx % cnt] + (size_t)(c0 + idx / cnt) * np];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int idx = tid + 256 * u;
+        if (idx < pairs) M[(size_t)s_dst[idx % cnt] + (size_t)(c0 + idx / cnt) * np] = tmp[u];
+    }
+}
+
+// Identity padding: M[i][i] = 1 for n <= i < np (the rest of the padding is already zero).
+template <class T>
+__global__ void pad_identity_kernel(cx<T>* __restrict__ A, int n, int np, long long ms) {
+    const int i = n + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) A[(size_t)blockIdx.y * ms + (size_t)i + (size_t)i * np] = cx<T>{T(1), T(0)};
+}
+
+// ------------------------------------------------------------------ launchers
+template <class T>
+void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
+                  hipStream_t st) {
+    const size_t shm = sizeof(int) * (size_t)np;
+    if (np <= 1024) {
+        hipLaunchKernelGGL((lu_panel_kernel<T, 8, 4>), dim3(nbatch), dim3(256), shm, st, A, np, mstride, ipiv, info,
+                           pmap, k0, nb);
+    } else if (np <= 2048) {
+        hipLaunchKernelGGL((lu_panel_kernel<T, 4, 8>), dim3(nbatch), dim3(256), shm, st, A, np, mstride, ipiv, info,
+                           pmap, k0, nb);
+    } else if (np <= 4096) {
+        hipLaunchKernelGGL((lu_panel_kernel<T, 2, 16>), dim3(nbatch), dim3(256), shm, st, A, np, mstride, ipiv,
+                           info, pmap, k0, nb);
+    } else {
+        hipLaunchKernelGGL((lu_panel_kernel<T, 1, 32>), dim3(nbatch), dim3(256), shm, st, A, np, mstride, ipiv,
+                           info, pmap, k0, nb);
+    }
+}
+
+template <class T>
+void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,
+                      int nbatch, hipStream_t st) {
+    const int kend = std::min(k0 + nb, np);
+    const int nleft_t = apply_left ? (k0 + 63) / 64 : 0;
+    const int nright_t = (np - kend + 63) / 64;
+    if (nleft_t + nright_t == 0) return;
+    hipLaunchKernelGGL(swap_trsm_kernel<T>, dim3(nleft_t + nright_t, nbatch), dim3(256), 0, st, A, np, mstride, pmap,
+                       k0, nb, nleft_t, 1);
+}
+
+template <class T>
+void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int k, int lda, int ldb, int ldc,
+                     long long sC, long long sA, long long sB, int nbatch, hipStream_t st) {
+    if (m <= 0 || n <= 0 || k <= 0) return;
+    const int tm = (m + GT - 1) / GT, tn = (n + GT - 1) / GT;
+    hipLaunchKernelGGL(gemm_sub_kernel<T>, dim3(tm * tn * nbatch), dim3(256), 0, st, C, A, B, m, n, k, lda, ldb, ldc,
+                       sC, sA, sB, tm, tn, nbatch);
+}
+
+template <class T>
+void launch_tracelog(const cx<T>* A, int n, int np, long long mstride, const int* ipiv, cx<double>* out,
+                     double* stats, int nbatch, hipStream_t st) {
+    hipLaunchKernelGGL(tracelog_kernel<T>, dim3(nbatch), dim3(256), 0, st, A, n, np, mstride, ipiv, out, stats);
+}
+
+void launch_build_perm(const int* pmap, int np, int nb, int* perm, int nbatch, hipStream_t st) {
+    hipLaunchKernelGGL(build_perm_kernel, dim3(nbatch), dim3(256), sizeof(int) * (size_t)np, st, pmap, np, nb, perm);
+}
+
+template <class T>
+void launch_gather_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, int ncol, long long sdst,
+                        long long ssrc, int nbatch, hipStream_t st) {
+    const long long tot = (long long)np * ncol;
+    hipLaunchKernelGGL(gather_rows_kernel<T>, dim3((unsigned)((tot + 255) / 256), 1, nbatch), dim3(256), 0, st, dst,
+                       src, perm, np, ncol, sdst, ssrc);
+}
+
+template <class T>
+void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int upper, long long slu, long long sb,
+                       int nbatch, hipStream_t st) {
+    dim3 grid((ncol + 63) / 64, nbatch);
+    if (upper)
+        hipLaunchKernelGGL((trsm_block_kernel<T, 1>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
+    else
+        hipLaunchKernelGGL((trsm_block_kernel<T, 0>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
+}
+
+template <class T>
+void launch_diag_sum(const cx<T>* X, int n, int ld, long long sx, cx<double>* out, int nbatch, hipStream_t st) {
+    hipLaunchKernelGGL(diag_sum_kernel<T>, dim3(nbatch), dim3(256), 0, st, X, n, ld, sx, out);
+}
+
+template <class T1, class T2>
+void launch_convert(const cx<T1>* x, cx<T2>* y, long long count, hipStream_t st) {
+    hipLaunchKernelGGL((convert_kernel<T1, T2>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, x, y, count);
+}
+
+void launch_residual(const cx<double>* M, const cx<double>* a, const cx<double>* phi, int np, cx<double>* r,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(residual_kernel, dim3((np + 63) / 64), dim3(64), 0, st, M, a, phi, np, r);
+}
+
+template <class T>
+void launch_axpy(cx<double>* a, const cx<T>* d, int n, hipStream_t st) {
+    hipLaunchKernelGGL(axpy_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, st, a, d, n);
+}
+
+
+template <class T>
+void launch_apply_map(cx<T>* X, int np, int ncol, const int* pmap, int nb, int q, long long sx, int nbatch,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(apply_map_kernel<T>, dim3((ncol + 7) / 8, nbatch), dim3(256), 0, st, X, np, ncol, pmap, nb, q,
+                       sx);
+}
+
+template <class T>
+void launch_swap_left(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int nbatch,
+                      hipStream_t st) {
+    const int nleft_t = (k0 + 63) / 64;
+    if (nleft_t == 0) return;
+    hipLaunchKernelGGL(swap_trsm_kernel<T>, dim3(nleft_t, nbatch), dim3(256), 0, st, A, np, mstride, pmap, k0, nb,
+                       nleft_t, 0);
+}
+
+template <class T>
+void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st) {
+    if (np <= n) return;
+    hipLaunchKernelGGL(pad_identity_kernel<T>, dim3((np - n + 63) / 64, nbatch), dim3(64), 0, st, A, n, np,
+                       (long long)np * np);
+}
+
+#define MSG_INST(T)                                                                                                   \
+    template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t);              \
+    template void launch_swap_trsm<T>(cx<T>*, int, long long, const int*, int, int, int, int, hipStream_t);           \
+    template void launch_gemm_sub<T>(cx<T>*, const cx<T>*, const cx<T>*, int, int, int, int, int, int, long long,      \
+                                     long long, long long, int, hipStream_t);                                          \
+    template void launch_tracelog<T>(const cx<T>*, int, int, long long, const int*, cx<double>*, double*, int,        \
+                                     hipStream_t);                                                                     \
+    template void launch_gather_rows<T>(cx<T>*, const cx<T>*, const int*, int, int, long long, long long, int,       \
+                                        hipStream_t);                                                                  \
+    template void launch_trsm_block<T>(const cx<T>*, cx<T>*, int, int, int, int, long long, long long, int,           \
+                                       hipStream_t);                                                                   \
+    template void launch_diag_sum<T>(const cx<T>*, int, int, long long, cx<double>*, int, hipStream_t);               \
+    template void launch_axpy<T>(cx<double>*, const cx<T>*, int, hipStream_t);                                       \
+    template void launch_apply_map<T>(cx<T>*, int, int, const int*, int, int, long long, int, hipStream_t);           \
+    template void launch_swap_left<T>(cx<T>*, int, long long, const int*, int, int, int, hipStream_t);                \
+    template void launch_pad_identity<T>(cx<T>*, int, int, int, hipStream_t);
+MSG_INST(double)
+MSG_INST(float)
+template void launch_convert<double, float>(const cx<double>*, cx<float>*, long long, hipStream_t);
+template void launch_convert<float, double>(const cx<float>*, cx<double>*, long long, hipStream_t);
+
+}  // namespace msg

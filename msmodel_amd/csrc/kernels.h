@@ -1,0 +1,62 @@
+// kernels.h -- launch wrappers of the HIP kernels (kern_*.hip), called by the
+// host engine.  All matrices are column-major interleaved complex, ld = np.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "dmath.h"
+
+namespace msg {
+
+// kern_assemble.hip
+void launch_diag(const cx<double>* ks, int nk, ScModel m, int d, int kind, cx<double>* diag, cx<double>* izero,
+                 hipStream_t st);
+template <class T>
+void launch_assemble(const double* pos, int n, int d, int np, const cx<double>* ks, const cx<double>* diag,
+                     const cx<double>* izero, int kind, cx<T>* out, long long mstride, int nbatch, hipStream_t st);
+void launch_incident(const double* pos, int n, int d, int np, cx<double> k, int source, double ct, double st_,
+                     cx<double>* phi, hipStream_t st);
+// pts == nullptr: nx x ny grid; otherwise nx points given row-major [nx][d] (ny ignored)
+void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int nx, int ny,
+                 double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi, hipStream_t st);
+
+// kern_lu.hip
+template <class T>
+void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
+                  hipStream_t st);
+template <class T>
+void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,
+                      int nbatch, hipStream_t st);
+template <class T>
+void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int k, int lda, int ldb, int ldc,
+                     long long sC, long long sA, long long sB, int nbatch, hipStream_t st);
+template <class T>
+void launch_tracelog(const cx<T>* A, int n, int np, long long mstride, const int* ipiv, cx<double>* out,
+                     double* stats, int nbatch, hipStream_t st);
+void launch_build_perm(const int* pmap, int np, int nb, int* perm, int nbatch, hipStream_t st);
+template <class T>
+void launch_gather_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, int ncol, long long sdst,
+                        long long ssrc, int nbatch, hipStream_t st);
+template <class T>
+void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int upper, long long slu, long long sb,
+                       int nbatch, hipStream_t st);
+template <class T>
+void launch_diag_sum(const cx<T>* X, int n, int ld, long long sx, cx<double>* out, int nbatch, hipStream_t st);
+template <class T1, class T2>
+void launch_convert(const cx<T1>* x, cx<T2>* y, long long count, hipStream_t st);
+void launch_residual(const cx<double>* M, const cx<double>* a, const cx<double>* phi, int np, cx<double>* r,
+                     hipStream_t st);
+template <class T>
+void launch_axpy(cx<double>* a, const cx<T>* d, int n, hipStream_t st);
+template <class T>
+void launch_apply_map(cx<T>* X, int np, int ncol, const int* pmap, int nb, int q, long long sx, int nbatch,
+                      hipStream_t st);
+template <class T>
+void launch_swap_left(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int nbatch,
+                      hipStream_t st);
+template <class T>
+void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st);
+
+// pmap ints per matrix for a given (np, nb)
+inline size_t pmap_ints(int np, int nb) { return (size_t)(np / nb) * (1 + 4 * (size_t)nb); }
+
+}  // namespace msg

@@ -236,6 +236,15 @@ def incident(source, k, x):
     return _z(lib().orc_incident(source, _c(k), x.size, _dp(x)))
 
 
+def total_wave(pos, a, k, x, source=PLANE):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n, d = pos.shape
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return _z(lib().orc_total_wave(source, _c(k), d, n, _dp(pos), a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   _dp(x)))
+
+
 def wavefunction_grid(pos, a, k, nx, ny, xmin, xmax, ymin, ymax, source=PLANE):
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     n, d = pos.shape
