@@ -127,12 +127,18 @@ int msgpu_profile_end(double* ms, double* flops, double* bytes, int* count, int 
 /* Trace-log of M(k) for nk wave numbers of one medium: out[l] = tracelog_lu(M(ks[l]))
  * as in run_tracelog_omp (ms_core.c:122-127).  mode = MSGPU_FP64/MIXED/FP32.
  * For MSGPU_MIXED, cells with N*Re(tracelog) < max_l N*Re(tracelog) + ln(reeval_rel)
- * (|det| below reeval_rel of the map maximum) and cells whose complex64 factor
- * has a pivot below pivot_tol*max|U_ii| are recomputed in FP64 (reeval_rel<=0
- * disables the first test).  flags[l] (may be NULL) = 1 for re-evaluated cells.
- * Returns 0 on success, <0 on error (message in msgpu_last_error). */
+ * (|det| below reeval_rel of the map maximum) and non-finite cells are recomputed in
+ * FP64 (reeval_rel<=0 disables the first test); the other cells keep the complex64
+ * value (typical per-N error 1e-7..1e-6 on dense media).  flags[l] (may be NULL) = 1
+ * for re-evaluated cells.  Returns the number of re-evaluated cells (>= 0) or <0 on
+ * error (message in msgpu_last_error). */
 int msgpu_tracelog_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode,
                          double reeval_rel, int* flags);
+/* As msgpu_tracelog_batch in FP64 or FP32 (no re-evaluation), with the pivot statistics
+ * stats[2l] = max_i log|U_ii|, stats[2l+1] = min_i log|U_ii| of cell l.  No counterpart in
+ * the reference; a conditioning indicator for the caller. */
+int msgpu_tracelog_stats(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, double* stats,
+                         int mode);
 /* d/dk log det M(k) = tr(M^-1 M') for nk wave numbers (ms_matrix_util.c:242-263). */
 int msgpu_dk_log_det_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode);
 /* Foldy-Lax amplitudes a = M(k)^-1 phi with phi the incident wave at the scatterers

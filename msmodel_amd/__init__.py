@@ -69,6 +69,7 @@ _SIG = {
     "msgpu_profile_end": (ctypes.c_int, [_dp, _dp, _dp, _ip, ctypes.c_int]),
     "msgpu_tracelog_batch": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, _ip]),
     "msgpu_dk_log_det_batch": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
+    "msgpu_tracelog_stats": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     "msgpu_solve": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_double]),
     "msgpu_wavefunction_grid": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -196,6 +197,14 @@ def tracelog_batch(med, ks, mode=FP64, reeval_rel=1e-6):
     _check(lib.msgpu_tracelog_batch(med.ref, ks.size, _cp(ks), _cp(out), mode, reeval_rel,
                                     flags.ctypes.data_as(_ip)), "msgpu_tracelog_batch")
     return out, flags
+
+
+def tracelog_stats(med, ks, mode=FP64):
+    ks = np.ascontiguousarray(ks, dtype=np.complex128)
+    out = np.zeros_like(ks)
+    st = np.zeros(2 * ks.size)
+    _check(lib.msgpu_tracelog_stats(med.ref, ks.size, _cp(ks), _cp(out), _cp(st), mode), "msgpu_tracelog_stats")
+    return out, st.reshape(-1, 2)
 
 
 def dk_log_det_batch(med, ks, mode=FP64):

@@ -96,16 +96,15 @@ MSGPU_API int msgpu_tracelog_batch(const msgpu_medium* med, int nk, const msgpu_
     int rc = tracelog_points(mv, nk, as_cx(ks), as_cx(out), stats.data(), 1);
     if (rc < 0) return rc;
     // Cells to recompute in FP64: |det| below reeval_rel of the map maximum (near zeros of
-    // det M, where the relative error of complex64 LU grows), and cells whose complex64
-    // factor has a pivot smaller than 1e-4 of its largest pivot (ill-conditioned M).
+    // det M, where the complex64 factor loses the most relative accuracy) and cells whose
+    // complex64 result is not finite.
     double best = -INFINITY;
-    for (int l = 0; l < nk; l++) best = std::max(best, mv.n * out[l].real());
+    for (int l = 0; l < nk; l++)
+        if (std::isfinite(out[l].real())) best = std::max(best, mv.n * out[l].real());
     const double cut = reeval_rel > 0 ? best + std::log(reeval_rel) : -INFINITY;
-    const double piv_cut = std::log(1e-4);
     std::vector<int> idx;
     for (int l = 0; l < nk; l++) {
-        const bool f = (mv.n * out[l].real() < cut) || (stats[2 * l + 1] - stats[2 * l] < piv_cut) ||
-                       !std::isfinite(out[l].real());
+        const bool f = (mv.n * out[l].real() < cut) || !std::isfinite(out[l].real());
         if (flags) flags[l] = f ? 1 : 0;
         if (f) idx.push_back(l);
     }
@@ -117,6 +116,11 @@ MSGPU_API int msgpu_tracelog_batch(const msgpu_medium* med, int nk, const msgpu_
         for (size_t i = 0; i < idx.size(); i++) out[idx[i]] = res[i];
     }
     return (int)idx.size();
+}
+
+MSGPU_API int msgpu_tracelog_stats(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out,
+                                   double* stats, int mode) {
+    return tracelog_points(view_of(med), nk, as_cx(ks), as_cx(out), stats, mode == MSGPU_FP64 ? 0 : 1);
 }
 
 MSGPU_API int msgpu_dk_log_det_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out,

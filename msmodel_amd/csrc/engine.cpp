@@ -84,7 +84,11 @@ template <class T> struct DBuf {
     T* p = nullptr;
     size_t cls = 0;
     DBuf() = default;
-    explicit DBuf(size_t count) { p = (T*)pool().get(sizeof(T) * count, cls); }
+    explicit DBuf(size_t count) { alloc(count); }
+    void alloc(size_t count) {
+        if (p) pool().put(p, cls);
+        p = (T*)pool().get(sizeof(T) * count, cls);
+    }
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() {
@@ -123,7 +127,11 @@ static StreamPool& streams() {
 struct Lease {
     hipStream_t st;
     Lease() : st(streams().get()) {}
-    ~Lease() { streams().put(st); }
+    explicit Lease(bool take) : st(take ? streams().get() : nullptr) {}
+    Lease(const Lease&) = delete;
+    Lease& operator=(const Lease&) = delete;
+    Lease(Lease&& o) noexcept : st(o.st) { o.st = nullptr; }
+    ~Lease() { if (st) streams().put(st); }
     void sync() { HIPCHK(hipStreamSynchronize(st)); }
 };
 
@@ -242,8 +250,12 @@ static int max_batch(int np, size_t elem, int mats_per_item) {
 }
 
 // ------------------------------------------------------------------ LU core
+// after_first_panel (optional) is recorded once the first panel is queued, so that a
+// concurrent stream can start its own factorization one panel phase later and the
+// latency-bound panels of one stream run under the GEMMs of the other.
 template <class T>
-static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, hipStream_t st) {
+static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, hipStream_t st,
+                   hipEvent_t after_first_panel = nullptr) {
     const long long ms = (long long)np * np;
     HIPCHK(hipMemsetAsync(info, 0, sizeof(int) * nbatch, st));
     const double es = sizeof(cx<T>);
@@ -251,6 +263,7 @@ static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* p
         const int kend = k0 + NB, rest = np - kend;
         timed(0, 8.0 * nbatch * ((double)(np - k0) * NB * NB / 2), es * nbatch * 2.0 * (np - k0) * NB, st,
               [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st); });
+        if (k0 == 0 && after_first_panel) HIPCHK(hipEventRecord(after_first_panel, st));
         timed(1, 8.0 * nbatch * (double)NB * NB / 2 * rest, es * nbatch * 2.0 * NB * rest, st,
               [&] { launch_swap_trsm<T>(mats, np, ms, pmap, k0, NB, left ? 1 : 0, nbatch, st); });
         if (kend < np)
@@ -288,34 +301,109 @@ static void lu_solve(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, co
 }
 
 // ------------------------------------------------------------------ batched trace-log
+// Chunking of a batch of nk items over NSTREAM streams: chunks of at most bmax items
+// (rounded to a multiple of 8 so the GEMM's XCD mapping applies), at least two chunks
+// when there is enough work, so that one chunk's latency-bound panel phases overlap
+// the other chunk's GEMMs.
+// NSTREAM streams (MSGPU_STREAMS overrides; the box exposes 4 hardware queues) and
+// chunks of at most MSGPU_CHUNK matrices.
+static constexpr int NSTREAM_MAX = 8;
+static int nstreams() {
+    static int v = [] {
+        const char* e = getenv("MSGPU_STREAMS");
+        int x = e ? atoi(e) : 4;
+        return std::max(1, std::min(x, NSTREAM_MAX));
+    }();
+    return v;
+}
+static int chunk_cap() {
+    static int v = [] {
+        const char* e = getenv("MSGPU_CHUNK");
+        int x = e ? atoi(e) : 256;
+        return std::max(8, std::min(x, 4096));
+    }();
+    return v;
+}
+static int chunk_size(int nk, int bmax) {
+    const int ns = nstreams();
+    bmax = std::min(bmax, chunk_cap());
+    int nch = (nk + bmax - 1) / bmax;
+    if (nch < ns && nk >= 32 * ns) nch = ns;
+    else if (nch < 2 && nk >= 64) nch = 2;
+    int c = (nk + nch - 1) / nch;
+    if (c >= 8) c = (c + 7) / 8 * 8;
+    return std::max(1, std::min(c, bmax));
+}
+
+struct Lanes {   // nstreams() leased streams; lane 0 owns the inputs, the others wait for them
+    int ns;
+    std::vector<Lease> l;
+    hipEvent_t ev[NSTREAM_MAX];
+    Lanes() : ns(nstreams()) {
+        l.reserve(ns);
+        for (int i = 0; i < ns; i++) l.emplace_back(true);
+        for (int i = 0; i < ns; i++) HIPCHK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    }
+    ~Lanes() { for (int i = 0; i < ns; i++) (void)hipEventDestroy(ev[i]); }
+    hipStream_t operator[](int i) const { return l[i].st; }
+    void fork() {
+        HIPCHK(hipEventRecord(ev[0], l[0].st));
+        for (int i = 1; i < ns; i++) HIPCHK(hipStreamWaitEvent(l[i].st, ev[0], 0));
+    }
+    void join() {
+        for (int i = 1; i < ns; i++) {
+            HIPCHK(hipEventRecord(ev[i], l[i].st));
+            HIPCHK(hipStreamWaitEvent(l[0].st, ev[i], 0));
+        }
+    }
+};
+
 template <class T>
 static void tracelog_impl(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, double* stats) {
-    Lease L;
+    Lanes S;
     const int np = mv.np;
+    const long long ms = (long long)np * np;
     DBuf<double> dpos((size_t)mv.n * mv.d);
     DBuf<cx<double>> dks(nk), ddiag(nk), dout(nk);
     DBuf<double> dstats(2 * (size_t)nk);
-    HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * mv.n * mv.d, hipMemcpyHostToDevice, L.st));
-    HIPCHK(hipMemcpyAsync(dks.p, ks, sizeof(cx<double>) * nk, hipMemcpyHostToDevice, L.st));
-    launch_diag(dks.p, nk, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
-    const int bmax = std::min(nk, max_batch(np, sizeof(cx<T>), 1));
-    DBuf<cx<T>> mats((size_t)bmax * np * np);
-    DBuf<int> ipiv((size_t)bmax * np), info(bmax), pmap((size_t)bmax * pmap_ints(np, NB));
-    for (int done = 0; done < nk; done += bmax) {
-        const int B = std::min(bmax, nk - done);
-        timed(3, 0.0, (double)B * np * np * sizeof(cx<T>), L.st, [&] {
-            launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag.p + done, nullptr, 0, mats.p,
-                               (long long)np * np, B, L.st);
+    HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * mv.n * mv.d, hipMemcpyHostToDevice, S[0]));
+    HIPCHK(hipMemcpyAsync(dks.p, ks, sizeof(cx<double>) * nk, hipMemcpyHostToDevice, S[0]));
+    launch_diag(dks.p, nk, mv.m, mv.d, 0, ddiag.p, nullptr, S[0]);
+    S.fork();
+    const int bsz = chunk_size(nk, max_batch(np, sizeof(cx<T>), S.ns));
+    const int nlane = std::min(S.ns, (nk + bsz - 1) / bsz);
+    std::vector<DBuf<cx<T>>> mats(nlane);
+    std::vector<DBuf<int>> ipiv(nlane), info(nlane), pmap(nlane);
+    for (int i = 0; i < nlane; i++) {
+        mats[i].alloc((size_t)bsz * ms);
+        ipiv[i].alloc((size_t)bsz * np);
+        info[i].alloc(bsz);
+        pmap[i].alloc((size_t)bsz * pmap_ints(np, NB));
+    }
+    // stagger: the first chunk of lane i starts after lane i-1 has finished its first panel
+    std::vector<hipEvent_t> stag(nlane, nullptr);
+    for (int i = 0; i + 1 < nlane; i++) HIPCHK(hipEventCreateWithFlags(&stag[i], hipEventDisableTiming));
+    int c = 0;
+    for (int done = 0; done < nk; done += bsz, c++) {
+        const int B = std::min(bsz, nk - done), ln = c % nlane;
+        hipStream_t st = S[ln];
+        if (c > 0 && c < nlane) HIPCHK(hipStreamWaitEvent(st, stag[c - 1], 0));
+        timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st, [&] {
+            launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag.p + done, nullptr, 0, mats[ln].p, ms, B,
+                               st);
         });
-        factor<T>(mats.p, np, B, ipiv.p, info.p, pmap.p, false, L.st);
-        timed(4, 0.0, (double)B * np * sizeof(cx<T>), L.st, [&] {
-            launch_tracelog<T>(mats.p, mv.n, np, (long long)np * np, ipiv.p, dout.p + done, dstats.p + 2 * done, B,
-                               L.st);
+        factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, st,
+                  (c + 1 < nlane) ? stag[c] : nullptr);
+        timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {
+            launch_tracelog<T>(mats[ln].p, mv.n, np, ms, ipiv[ln].p, dout.p + done, dstats.p + 2 * done, B, st);
         });
     }
-    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, L.st));
-    if (stats) HIPCHK(hipMemcpyAsync(stats, dstats.p, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, L.st));
-    L.sync();
+    S.join();
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, S[0]));
+    if (stats) HIPCHK(hipMemcpyAsync(stats, dstats.p, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, S[0]));
+    S.l[0].sync();
+    for (hipEvent_t e : stag)
+        if (e) (void)hipEventDestroy(e);
 }
 
 int tracelog_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, double* stats,
@@ -332,28 +420,40 @@ int tracelog_points(const MediumView& mv, int nk, const cx<double>* ks, cx<doubl
 // ------------------------------------------------------------------ batched d/dk log det
 template <class T>
 static void dk_impl(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out) {
-    Lease L;
+    Lanes S;
     const int np = mv.np;
+    const long long ms = (long long)np * np;
     DBuf<double> dpos((size_t)mv.n * mv.d);
     DBuf<cx<double>> dks(nk), ddiag(nk), ddiag1(nk), dout(nk);
-    HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * mv.n * mv.d, hipMemcpyHostToDevice, L.st));
-    HIPCHK(hipMemcpyAsync(dks.p, ks, sizeof(cx<double>) * nk, hipMemcpyHostToDevice, L.st));
-    launch_diag(dks.p, nk, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
-    launch_diag(dks.p, nk, mv.m, mv.d, 1, ddiag1.p, nullptr, L.st);
-    const int bmax = std::min(nk, max_batch(np, sizeof(cx<T>), 2));
-    const long long ms = (long long)np * np;
-    DBuf<cx<T>> mats((size_t)bmax * ms), rhs((size_t)bmax * ms);
-    DBuf<int> ipiv((size_t)bmax * np), info(bmax), pmap((size_t)bmax * pmap_ints(np, NB));
-    for (int done = 0; done < nk; done += bmax) {
-        const int B = std::min(bmax, nk - done);
-        launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag.p + done, nullptr, 0, mats.p, ms, B, L.st);
-        launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag1.p + done, nullptr, 1, rhs.p, ms, B, L.st);
-        factor<T>(mats.p, np, B, ipiv.p, info.p, pmap.p, false, L.st);
-        lu_solve<T>(mats.p, rhs.p, np, np, B, pmap.p, ms, L.st);
-        launch_diag_sum<T>(rhs.p, mv.n, np, ms, dout.p + done, B, L.st);
+    HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * mv.n * mv.d, hipMemcpyHostToDevice, S[0]));
+    HIPCHK(hipMemcpyAsync(dks.p, ks, sizeof(cx<double>) * nk, hipMemcpyHostToDevice, S[0]));
+    launch_diag(dks.p, nk, mv.m, mv.d, 0, ddiag.p, nullptr, S[0]);
+    launch_diag(dks.p, nk, mv.m, mv.d, 1, ddiag1.p, nullptr, S[0]);
+    S.fork();
+    const int bsz = chunk_size(nk, max_batch(np, sizeof(cx<T>), 2 * S.ns));
+    const int nlane = std::min(S.ns, (nk + bsz - 1) / bsz);
+    std::vector<DBuf<cx<T>>> mats(nlane), rhs(nlane);
+    std::vector<DBuf<int>> ipiv(nlane), info(nlane), pmap(nlane);
+    for (int i = 0; i < nlane; i++) {
+        mats[i].alloc((size_t)bsz * ms);
+        rhs[i].alloc((size_t)bsz * ms);
+        ipiv[i].alloc((size_t)bsz * np);
+        info[i].alloc(bsz);
+        pmap[i].alloc((size_t)bsz * pmap_ints(np, NB));
     }
-    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, L.st));
-    L.sync();
+    int c = 0;
+    for (int done = 0; done < nk; done += bsz, c++) {
+        const int B = std::min(bsz, nk - done), ln = c % nlane;
+        hipStream_t st = S[ln];
+        launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag.p + done, nullptr, 0, mats[ln].p, ms, B, st);
+        launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag1.p + done, nullptr, 1, rhs[ln].p, ms, B, st);
+        factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, st);
+        lu_solve<T>(mats[ln].p, rhs[ln].p, np, np, B, pmap[ln].p, ms, st);
+        launch_diag_sum<T>(rhs[ln].p, mv.n, np, ms, dout.p + done, B, st);
+    }
+    S.join();
+    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, S[0]));
+    S.l[0].sync();
 }
 
 int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, int precision) {

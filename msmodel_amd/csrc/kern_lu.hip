@@ -25,33 +25,39 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 template <class T> struct mfma;
 // Operands (both types): lane l supplies A[l%16][l/16] and B[l/16][l%16].
-// Result: lane l, register r holds D[row(l,r)][l%16], where the row mapping differs
-// between the f64 and the f32 instruction (checked on gfx950 with an in-bounds probe).
+// Result: lane l, register r holds D[i][l%16] with i = 4*(l/16)+r for f32 and
+// i = l/16 + 4*r for f64 (checked on gfx950 with an in-bounds probe).  For f64 the
+// GEMM feeds MFMA row i with real row arow(i) = 4*(i%4) + i/4 of the tile, so that
+// for both types the four registers of a lane are real rows 4*(l/16)..+3, i.e. two
+// consecutive complex elements of one column (one 32- or 16-byte access).
 template <> struct mfma<double> {
     using acc = d4v;
     static __device__ __forceinline__ acc op(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+    static __device__ __forceinline__ int arow(int i) { return 4 * (i & 3) + (i >> 2); }
 };
 template <> struct mfma<float> {
     using acc = f4v;
     static __device__ __forceinline__ acc op(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+    static __device__ __forceinline__ int arow(int i) { return i; }
 };
 
 // ------------------------------------------------------------------ panel
 // pmap layout per matrix and panel q: [cnt, dst[2*nb], src[2*nb]] (1 + 4*nb ints).
-template <class T, int S, int R>
-__global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, int np, long long mstride,
-                                                       int* __restrict__ ipiv_all, int* __restrict__ info_all,
-                                                       int* __restrict__ pmap_all, int k0, int nb) {
+// One 1024-thread workgroup per matrix; row g of the current sub-panel (columns
+// [c0, c0+S)) lives in thread (g-c0) % PT, register row (g-c0) / PT.
+template <class T, int S, int R, int PT>
+__global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int np, long long mstride,
+                                                      int* __restrict__ ipiv_all, int* __restrict__ info_all,
+                                                      int* __restrict__ pmap_all, int k0, int nb) {
+    constexpr int PW = PT / 64;
     extern __shared__ int s_slot[];                 // np ints: row -> slot of the composite map
-    __shared__ T s_abs[2][4];
-    __shared__ int s_idx[2][4];
-    __shared__ cx<T> s_row[2][4][S];
+    __shared__ T s_abs[2][PW];
+    __shared__ int s_idx[2][PW];
+    __shared__ cx<T> s_row[2][PW][S];
     __shared__ cx<T> s_rowj[2][S];
     __shared__ cx<T> s_L[S][S];
     __shared__ cx<T> s_U[S][64];
@@ -66,7 +72,7 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, in
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int kend = min(k0 + nb, np);
 
-    for (int i = tid; i < np; i += 256) s_slot[i] = -1;
+    for (int i = tid; i < np; i += PT) s_slot[i] = -1;
     if (tid == 0) s_cnt = 0;
     __syncthreads();
 
@@ -74,21 +80,24 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, in
         cx<T> x[R][S];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int g = c0 + tid + 256 * r;
+            const int g = c0 + tid + PT * r;
 #pragma unroll
             for (int c = 0; c < S; c++)
                 x[r][c] = (g < np) ? M[(size_t)g + (size_t)(c0 + c) * np] : cx<T>{T(0), T(0)};
         }
-#pragma unroll
+        // Column loop with a rotating register file: at step jj, x[r][c] holds column
+        // c0 + (jj + c) % S, so the active column is always x[r][0] and every register
+        // index is a compile-time constant while the loop stays rolled.
+#pragma unroll 1
         for (int jj = 0; jj < S; jj++) {
             const int j = c0 + jj, buf = jj & 1;
             T best = T(-1);
             int bidx = 0x7fffffff, br = -1;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const int g = c0 + tid + 256 * r;
+                const int g = c0 + tid + PT * r;
                 if (g >= j && g < np) {
-                    T v = abs1(x[r][jj]);
+                    T v = abs1(x[r][0]);
                     if (v > best) { best = v; bidx = g; br = r; }
                 }
             }
@@ -105,28 +114,24 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, in
                 for (int r = 0; r < R; r++)
                     if (r == br)
 #pragma unroll
-                        for (int c = 0; c < S; c++) s_row[buf][wv][c] = x[r][c];
+                        for (int c = 0; c < S; c++) s_row[buf][wv][(jj + c) % S] = x[r][c];
             }
             if (tid == jj) {        // row j = c0 + jj lives in thread jj, register row 0
 #pragma unroll
-                for (int c = 0; c < S; c++) s_rowj[buf][c] = x[0][c];
+                for (int c = 0; c < S; c++) s_rowj[buf][(jj + c) % S] = x[0][c];
             }
             __syncthreads();
             int bw = 0;
-#pragma unroll
-            for (int w = 1; w < 4; w++)
+            for (int w = 1; w < PW; w++)
                 if (s_abs[buf][w] > s_abs[buf][bw] || (s_abs[buf][w] == s_abs[buf][bw] && s_idx[buf][w] < s_idx[buf][bw]))
                     bw = w;
             const int p = s_idx[buf][bw];
-            cx<T> piv[S];
-#pragma unroll
-            for (int c = 0; c < S; c++) piv[c] = s_row[buf][bw][c];
+            const cx<T>* piv = s_row[buf][bw];
             const cx<T> pv = piv[jj];
             const bool zero = (pv.x == T(0) && pv.y == T(0));
             if (tid == 0) {
                 ipiv[j] = p + 1;
                 if (zero && info_all[b] == 0) info_all[b] = j + 1;
-                // composite map of the panel's interchanges
                 int sj = s_slot[j];
                 if (sj < 0) { sj = s_cnt++; s_slot[j] = sj; s_dst[sj] = j; s_src[sj] = j; }
                 int sp = s_slot[p];
@@ -134,35 +139,43 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, in
                 int t = s_src[sj]; s_src[sj] = s_src[sp]; s_src[sp] = t;
             }
             if (p != j) {
-                const int op = (p - c0) & 255, rp = (p - c0) >> 8;
+                const int op = (p - c0) % PT, rp = (p - c0) / PT;
                 if (tid == op) {
 #pragma unroll
                     for (int r = 0; r < R; r++)
                         if (r == rp)
 #pragma unroll
-                            for (int c = 0; c < S; c++) x[r][c] = s_rowj[buf][c];
+                            for (int c = 0; c < S; c++) x[r][c] = s_rowj[buf][(jj + c) % S];
                 }
                 if (tid == jj) {
 #pragma unroll
-                    for (int c = 0; c < S; c++) x[0][c] = piv[c];
+                    for (int c = 0; c < S; c++) x[0][c] = piv[(jj + c) % S];
                 }
             }
             const cx<T> rinv = zero ? cx<T>{T(1), T(0)} : inv(pv);
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const int g = c0 + tid + 256 * r;
+                const int g = c0 + tid + PT * r;
                 if (g > j && g < np) {
-                    const cx<T> l = x[r][jj] * rinv;
-                    x[r][jj] = l;
+                    const cx<T> l = x[r][0] * rinv;
+                    x[r][0] = l;
 #pragma unroll
-                    for (int c = jj + 1; c < S; c++) x[r][c] = x[r][c] - l * piv[c];
+                    for (int c = 1; c < S; c++)
+                        if (jj + c < S) x[r][c] = x[r][c] - l * piv[jj + c];
                 }
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const cx<T> t = x[r][0];
+#pragma unroll
+                for (int c = 0; c < S - 1; c++) x[r][c] = x[r][c + 1];
+                x[r][S - 1] = t;
             }
         }
         // write the factored sub-panel back
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int g = c0 + tid + 256 * r;
+            const int g = c0 + tid + PT * r;
             if (g < np)
 #pragma unroll
                 for (int c = 0; c < S; c++) M[(size_t)g + (size_t)(c0 + c) * np] = x[r][c];
@@ -193,7 +206,7 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, in
             cx<T> tmp[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int idx = tid + 256 * u;
+                const int idx = tid + PT * u;
                 if (idx < pairs) {
                     const int e = idx % cnt, ci = idx / cnt;
                     const int col = ci < nleft ? k0 + ci : c0 + S + (ci - nleft);
@@ -203,7 +216,7 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, in
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int idx = tid + 256 * u;
+                const int idx = tid + PT * u;
                 if (idx < pairs) {
                     const int e = idx % cnt, ci = idx / cnt;
                     const int col = ci < nleft ? k0 + ci : c0 + S + (ci - nleft);
@@ -214,48 +227,35 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, in
         }
         if (nright > 0) {
             // U12' = L11'^-1 A12' for the panel columns right of the sub-panel
-            if (tid < nright) {
+            if (tid < nright) {          // solved in LDS: the registers hold the sub-panel
                 const int col = c0 + S + tid;
-                cx<T> u[S];
-#pragma unroll
-                for (int i = 0; i < S; i++) u[i] = M[(size_t)(c0 + i) + (size_t)col * np];
-#pragma unroll
-                for (int i = 1; i < S; i++)
-#pragma unroll
-                    for (int m = 0; m < i; m++) u[i] = u[i] - s_L[i][m] * u[m];
-#pragma unroll
-                for (int i = 0; i < S; i++) {
-                    M[(size_t)(c0 + i) + (size_t)col * np] = u[i];
-                    s_U[i][tid] = u[i];
+                for (int i = 0; i < S; i++) s_U[i][tid] = M[(size_t)(c0 + i) + (size_t)col * np];
+                for (int i = 1; i < S; i++) {
+                    cx<T> acc = s_U[i][tid];
+                    for (int m = 0; m < i; m++) acc = acc - s_L[i][m] * s_U[m][tid];
+                    s_U[i][tid] = acc;
                 }
+                for (int i = 0; i < S; i++) M[(size_t)(c0 + i) + (size_t)col * np] = s_U[i][tid];
             }
             __syncthreads();
-            // rank-S update of rows below the sub-panel: WR columns x RG register rows per round
+            // rank-S update of rows below the sub-panel, WR columns per round
             constexpr int WR = sizeof(T) == 8 ? 4 : 8;
-            constexpr int RG = (R * WR * (int)sizeof(T) <= 128) ? R : (128 / (WR * (int)sizeof(T)));
             for (int w0 = 0; w0 < nright; w0 += WR) {
 #pragma unroll
-                for (int r0 = 0; r0 < R; r0 += RG) {
-                    cx<T> av[RG][WR];
+                for (int r = 0; r < R; r++) {
+                    const int g = c0 + tid + PT * r;
+                    if (g < c0 + S || g >= np) continue;
+                    cx<T> av[WR];
 #pragma unroll
-                    for (int rr = 0; rr < RG; rr++) {
-                        const int g = c0 + tid + 256 * (r0 + rr);
+                    for (int w = 0; w < WR; w++)
+                        if (w0 + w < nright) av[w] = M[(size_t)g + (size_t)(c0 + S + w0 + w) * np];
 #pragma unroll
-                        for (int w = 0; w < WR; w++)
-                            if (g >= c0 + S && g < np && w0 + w < nright)
-                                av[rr][w] = M[(size_t)g + (size_t)(c0 + S + w0 + w) * np];
-                    }
+                    for (int w = 0; w < WR; w++) {
+                        if (w0 + w < nright) {
+                            cx<T> a = av[w];
 #pragma unroll
-                    for (int rr = 0; rr < RG; rr++) {
-                        const int g = c0 + tid + 256 * (r0 + rr);
-#pragma unroll
-                        for (int w = 0; w < WR; w++) {
-                            if (g >= c0 + S && g < np && w0 + w < nright) {
-                                cx<T> a = av[rr][w];
-#pragma unroll
-                                for (int m = 0; m < S; m++) a = a - x[r0 + rr][m] * s_U[m][w0 + w];
-                                M[(size_t)g + (size_t)(c0 + S + w0 + w) * np] = a;
-                            }
+                            for (int m = 0; m < S; m++) a = a - x[r][m] * s_U[m][w0 + w];
+                            M[(size_t)g + (size_t)(c0 + S + w0 + w) * np] = a;
                         }
                     }
                 }
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(cx<T>* __restrict__ A, in
     __syncthreads();
     const int cnt = s_cnt;
     if (tid == 0) pmap[0] = cnt;
-    for (int e = tid; e < cnt; e += 256) {
+    for (int e = tid; e < cnt; e += PT) {
         pmap[1 + e] = s_dst[e];
         pmap[1 + 2 * nb + e] = s_src[e];
     }
@@ -324,8 +324,10 @@ __device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], const cx<T> (*sL)[6
 }
 
 // ------------------------------------------------------------------ swap + TRSM
-// grid.x = column tiles: the first nleft_t tiles cover [0, k0) (LAPACK-format L),
-// the rest cover [kend, ncols).  Right tiles also get U12 = L11^-1 A12.
+// grid.x = column tiles of SWT columns: the first nleft_t tiles cover [0, k0)
+// (LAPACK-format L), the rest cover [kend, np).  Right tiles also get
+// U12 = L11^-1 A12, 64 columns at a time against L11 staged once in LDS.
+constexpr int SWT = 256;
 template <class T>
 __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, int np, long long mstride,
                                                         const int* __restrict__ pmap_all, int k0, int nb,
@@ -339,8 +341,8 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
     const int kend = min(k0 + nb, np);
     const int tid = threadIdx.x;
     const bool left = (int)blockIdx.x < nleft_t;
-    const int col0 = left ? blockIdx.x * 64 : kend + (blockIdx.x - nleft_t) * 64;
-    const int colend = left ? min(col0 + 64, k0) : min(col0 + 64, np);
+    const int col0 = left ? blockIdx.x * SWT : kend + (blockIdx.x - nleft_t) * SWT;
+    const int colend = left ? min(col0 + SWT, k0) : min(col0 + SWT, np);
     const int ncol = colend - col0;
     const int cnt = pmap[0];
     for (int e = tid; e < cnt; e += 256) { s_dst[e] = pmap[1 + e]; s_src[e] = pmap[1 + 2 * nb + e]; }
@@ -378,21 +380,22 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
     if (left || !do_trsm) return;
     // TRSM: 4 lanes per column; lane q of a group holds rows i = 4m + q, m < nb/4
     const int cl = tid >> 2, qq = tid & 3, lane = tid & 63;
-    const int col = col0 + cl;
-    const bool live = cl < ncol;
-    constexpr int MR = 16;               // nb <= 64
-    cx<T> xv[MR];
+    for (int sub = 0; sub < ncol; sub += 64) {
+        const int col = col0 + sub + cl;
+        const bool live = sub + cl < ncol;
+        cx<T> xv[16];
 #pragma unroll
-    for (int m = 0; m < MR; m++) {
-        const int i = 4 * m + qq;
-        xv[m] = (live && i < nb) ? M[(size_t)(k0 + i) + (size_t)col * np] : cx<T>{T(0), T(0)};
-    }
-    lower_unit_solve<T>(xv, sL, nb, qq, lane);
-    if (live) {
-#pragma unroll
-        for (int m = 0; m < MR; m++) {
+        for (int m = 0; m < 16; m++) {
             const int i = 4 * m + qq;
-            if (i < nb) M[(size_t)(k0 + i) + (size_t)col * np] = xv[m];
+            xv[m] = (live && i < nb) ? M[(size_t)(k0 + i) + (size_t)col * np] : cx<T>{T(0), T(0)};
+        }
+        lower_unit_solve<T>(xv, sL, nb, qq, lane);
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const int i = 4 * m + qq;
+                if (i < nb) M[(size_t)(k0 + i) + (size_t)col * np] = xv[m];
+            }
         }
     }
 }
@@ -424,50 +427,61 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(cx<T>* C, const cx<T>* A,
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w & 1, wc = w >> 1;
     using acc_t = typename mfma<T>::acc;
+    constexpr int LU = KC * GT / 256;     // complex elements per thread per operand and chunk
+    // prefetch registers for the next K chunk
+    cx<T> pa[LU], pb[LU];
+    auto fetch = [&](int kc0) {
+#pragma unroll
+        for (int u = 0; u < LU; u++) {
+            const int e = tid + 256 * u;
+            const int kcol = e / GT, row = e % GT;
+            pa[u] = (row0 + row < m && kc0 + kcol < kk) ? A[(size_t)(row0 + row) + (size_t)(kc0 + kcol) * lda]
+                                                       : cx<T>{T(0), T(0)};
+            const int col = e / KC, kr = e % KC;
+            pb[u] = (col0 + col < n && kc0 + kr < kk) ? B[(size_t)(kc0 + kr) + (size_t)(col0 + col) * ldb]
+                                                     : cx<T>{T(0), T(0)};
+        }
+    };
+    fetch(0);
+    // C tile into the accumulators: lane holds complex rows crow, crow+1 of column ccol
     acc_t acc[4][2];
-    // load C into the MFMA output layout (one real component per register)
-    T* Ct = reinterpret_cast<T*>(C);
 #pragma unroll
     for (int rb = 0; rb < 4; rb++)
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {
+            const int crow = row0 + wr * 32 + rb * 8 + 2 * (lane >> 4);
             const int ccol = col0 + wc * 32 + cb * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int rr = wr * 64 + rb * 16 + mfma<T>::row(lane, r);
-                const int crow = row0 + (rr >> 1);
-                acc[rb][cb][r] = (ccol < n && crow < m) ? Ct[2 * ((size_t)crow + (size_t)ccol * ldc) + (rr & 1)] : T(0);
+            cx<T> v0{T(0), T(0)}, v1{T(0), T(0)};
+            if (ccol < n) {
+                const cx<T>* cp = C + (size_t)crow + (size_t)ccol * ldc;
+                if (crow + 1 < m) { v0 = cp[0]; v1 = cp[1]; }
+                else if (crow < m) v0 = cp[0];
             }
+            acc[rb][cb][0] = v0.x; acc[rb][cb][1] = v0.y; acc[rb][cb][2] = v1.x; acc[rb][cb][3] = v1.y;
         }
+    const int kq = lane >> 4, rot = kq & 1;
+    const int ar = mfma<T>::arow(lane & 15);
     for (int kc0 = 0; kc0 < kk; kc0 += KC) {
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < KC * GT / 256; u++) {
+        for (int u = 0; u < LU; u++) {
             const int e = tid + 256 * u;
             const int kcol = e / GT, row = e % GT;
-            cx<T> v{T(0), T(0)};
-            if (row0 + row < m && kc0 + kcol < kk) v = A[(size_t)(row0 + row) + (size_t)(kc0 + kcol) * lda];
-            lA[kcol][2 * row] = v.x;
-            lA[kcol][2 * row + 1] = v.y;
-        }
-#pragma unroll
-        for (int u = 0; u < KC * GT / 256; u++) {
-            const int e = tid + 256 * u;
+            lA[kcol][2 * row] = pa[u].x;
+            lA[kcol][2 * row + 1] = pa[u].y;
             const int col = e / KC, kr = e % KC;
-            cx<T> v{T(0), T(0)};
-            if (col0 + col < n && kc0 + kr < kk) v = B[(size_t)(kc0 + kr) + (size_t)(col0 + col) * ldb];
-            lB[col][2 * kr] = v.x;
-            lB[col][2 * kr + 1] = v.y;
+            lB[col][2 * kr] = pb[u].x;
+            lB[col][2 * kr + 1] = pb[u].y;
         }
         __syncthreads();
-        const int kq = lane >> 4, rot = kq & 1;
+        if (kc0 + KC < kk) fetch(kc0 + KC);     // in flight while this chunk computes
 #pragma unroll
         for (int ks = 0; ks < 2 * KC; ks += 4) {
             const int p = (ks >> 1) + (kq >> 1);
             T av[4], bv[2];
 #pragma unroll
             for (int rb = 0; rb < 4; rb++) {
-                const int rr = wr * 64 + rb * 16 + (lane & 15);
+                const int rr = wr * 64 + rb * 16 + ar;
                 const int comp = rr & 1;
                 T v;
                 if (!rot) v = -lA[p][rr];
@@ -486,12 +500,12 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(cx<T>* C, const cx<T>* A,
     for (int rb = 0; rb < 4; rb++)
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {
+            const int crow = row0 + wr * 32 + rb * 8 + 2 * (lane >> 4);
             const int ccol = col0 + wc * 32 + cb * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int rr = wr * 64 + rb * 16 + mfma<T>::row(lane, r);
-                const int crow = row0 + (rr >> 1);
-                if (ccol < n && crow < m) Ct[2 * ((size_t)crow + (size_t)ccol * ldc) + (rr & 1)] = acc[rb][cb][r];
+            if (ccol < n) {
+                cx<T>* cp = C + (size_t)crow + (size_t)ccol * ldc;
+                if (crow < m) cp[0] = cx<T>{acc[rb][cb][0], acc[rb][cb][1]};
+                if (crow + 1 < m) cp[1] = cx<T>{acc[rb][cb][2], acc[rb][cb][3]};
             }
         }
 }
@@ -684,22 +698,29 @@ __global__ void pad_identity_kernel(cx<T>* __restrict__ A, int n, int np, long l
 }
 
 // ------------------------------------------------------------------ launchers
+// Panel shapes (threads PT, sub-panel width S, register rows R = rows / PT), chosen
+// per precision and padded size from measurements of the panel microbenchmark.
+template <class T, int S, int R, int PT>
+static void panel_go(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb,
+                     int nbatch, hipStream_t st) {
+    hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT>), dim3(nbatch), dim3(PT), sizeof(int) * (size_t)np, st, A, np,
+                       mstride, ipiv, info, pmap, k0, nb);
+}
+
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
                   hipStream_t st) {
-    const size_t shm = sizeof(int) * (size_t)np;
+    constexpr bool D = sizeof(T) == 8;
     if (np <= 1024) {
-        hipLaunchKernelGGL((lu_panel_kernel<T, 8, 4>), dim3(nbatch), dim3(256), shm, st, A, np, mstride, ipiv, info,
-                           pmap, k0, nb);
+        if (D) panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
+        else panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
     } else if (np <= 2048) {
-        hipLaunchKernelGGL((lu_panel_kernel<T, 4, 8>), dim3(nbatch), dim3(256), shm, st, A, np, mstride, ipiv, info,
-                           pmap, k0, nb);
+        if (D) panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
+        else panel_go<T, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
     } else if (np <= 4096) {
-        hipLaunchKernelGGL((lu_panel_kernel<T, 2, 16>), dim3(nbatch), dim3(256), shm, st, A, np, mstride, ipiv,
-                           info, pmap, k0, nb);
+        panel_go<T, 4, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
     } else {
-        hipLaunchKernelGGL((lu_panel_kernel<T, 1, 32>), dim3(nbatch), dim3(256), shm, st, A, np, mstride, ipiv,
-                           info, pmap, k0, nb);
+        panel_go<T, 2, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
     }
 }
 
@@ -707,8 +728,8 @@ template <class T>
 void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,
                       int nbatch, hipStream_t st) {
     const int kend = std::min(k0 + nb, np);
-    const int nleft_t = apply_left ? (k0 + 63) / 64 : 0;
-    const int nright_t = (np - kend + 63) / 64;
+    const int nleft_t = apply_left ? (k0 + SWT - 1) / SWT : 0;
+    const int nright_t = (np - kend + SWT - 1) / SWT;
     if (nleft_t + nright_t == 0) return;
     hipLaunchKernelGGL(swap_trsm_kernel<T>, dim3(nleft_t + nright_t, nbatch), dim3(256), 0, st, A, np, mstride, pmap,
                        k0, nb, nleft_t, 1);
@@ -782,7 +803,7 @@ void launch_apply_map(cx<T>* X, int np, int ncol, const int* pmap, int nb, int q
 template <class T>
 void launch_swap_left(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int nbatch,
                       hipStream_t st) {
-    const int nleft_t = (k0 + 63) / 64;
+    const int nleft_t = (k0 + SWT - 1) / SWT;
     if (nleft_t == 0) return;
     hipLaunchKernelGGL(swap_trsm_kernel<T>, dim3(nleft_t, nbatch), dim3(256), 0, st, A, np, mstride, pmap, k0, nb,
                        nleft_t, 0);
