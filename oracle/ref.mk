@@ -29,6 +29,9 @@ ALL_OBJ  := $(patsubst $(SRC)/%.c,$(OBJ)/%.o,$(ALL_SRC))
 
 all: $(OUT)/libmsref.so $(OUT)/libmsref_shim.so $(OUT)/msmain
 
+# msmain_gpu needs msmodel_amd/libmsgpu.so (python msmodel_amd/build.py); not part of `all`.
+gpu: $(OUT)/msmain_gpu
+
 $(OBJ):
 	mkdir -p $@
 
@@ -47,7 +50,25 @@ $(OUT)/libmsref_shim.so: oracle/ref_shim.c $(OUT)/libmsref.so
 $(OUT)/msmain: $(ALL_OBJ) $(OBJ)/ms_main.o
 	$(CC) -fopenmp $^ $(LDLIBS) -o $@
 
+# Drop-in demonstration: the reference's own msmain with the Foldy-Lax path served by
+# libmsgpu.so.  The replaced functions are made local in copies of the two reference
+# objects that define them (their other functions stay), so every call from ms_core.c
+# binds to the library; -lmsgpu precedes MKL so zgesv_/zgetrf_/zgetrs_ bind to it too.
+# MKL is named by path: -L$(MKLDIR) would put conda's older libstdc++ in front of the
+# one the HIP runtime needs.
+DROPIN   := build_ms_matrix build_dk_ms_matrix build_normalized_matrix build_plane_wave \
+            tracelog_lu dk_log_det_ms traceinv inverse_iteration total_cross_section \
+            eval_total_wave eval_incident_wave
+GPU_OBJ  := $(filter-out $(OBJ)/ms_matrix_util.o $(OBJ)/incident_wave_util.o,$(ALL_OBJ))
+
+$(OBJ)/%.local.o: $(OBJ)/%.o
+	objcopy $(foreach f,$(DROPIN),--localize-symbol=$(f)) $< $@
+
+$(OUT)/msmain_gpu: $(GPU_OBJ) $(OBJ)/ms_matrix_util.local.o $(OBJ)/incident_wave_util.local.o $(OBJ)/ms_main.o msmodel_amd/libmsgpu.so
+	$(CC) -fopenmp $(filter %.o,$^) -Lmsmodel_amd -lmsgpu -Wl,-rpath,'$$ORIGIN/../../msmodel_amd' \
+	    -Wl,-rpath-link,/usr/lib/x86_64-linux-gnu -lm $(MKLDIR)/libmkl_rt.so.1 -Wl,-rpath,$(MKLDIR) -o $@
+
 clean:
 	rm -rf $(OUT)
 
-.PHONY: all clean
+.PHONY: all gpu clean
