@@ -114,10 +114,23 @@ def main():
     ap.add_argument("--plain", type=int, default=24, help="cells timed on the one-matrix-per-call FP64 path")
     ap.add_argument("--requests", type=int, default=12, help="resonance requests for p50/p99")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run this many device copies of 1 GiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs on one GPU; --gpus %d is not supported (the k points of a map "
                  "shard across GPUs without exchange, see DESIGN.md)" % args.gpus)
+
+    if args.copy_probe:
+        import torch
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        for _ in range(args.copy_probe):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8, "copies": args.copy_probe}))
+        return
 
     import msmodel_amd as G
     G.init()

@@ -151,6 +151,26 @@ int msgpu_solve(const msgpu_medium* med, const msgpu_iwave* iwave, msgpu_cplx* a
 int msgpu_wavefunction_grid(const msgpu_medium* med, const msgpu_iwave* iwave, const msgpu_cplx* a, int nx,
                             int ny, double xmin, double xmax, double ymin, double ymax, msgpu_cplx* psi);
 
+/* Batches over several media.  No single counterpart in the reference: its task
+ * drivers loop over seeds and media one matrix at a time (run_tracelog_omp,
+ * ms_core.c:96-148; find_eigenstates_serial_1, ms_core.c:320-327).  Item i is the
+ * wave number ks[i] on medium meds[med_of[i]] (med_of may be NULL: all on meds[0]).
+ * Media may differ in N, d and scattering model; items are grouped by padded size
+ * and each result is the one the single-medium call gives for that item.
+ * MSGPU_MIXED re-evaluates, per medium, the cells below reeval_rel of that medium's
+ * largest |det| in the batch. */
+int msgpu_tracelog_multi(int nmed, const msgpu_medium* const* meds, int nk, const int* med_of,
+                         const msgpu_cplx* ks, msgpu_cplx* out, int mode, double reeval_rel, int* flags);
+int msgpu_dk_log_det_multi(int nmed, const msgpu_medium* const* meds, int nk, const int* med_of,
+                           const msgpu_cplx* ks, msgpu_cplx* out, int mode);
+/* Resonances of several media at once (the uneven-batch search): medium m searches
+ * the open rectangle doms[4m..4m+3] = {xmin, xmax, ymin, ymax} from nz[m] starts
+ * stored in z at offset sum_{j<m} nz[j]; on return nz[m] roots of medium m are at
+ * that same offset, sorted by real part.  Every sweep evaluates f'/f for the live
+ * starts of all media in one batched call per precision.  Returns the sweep count. */
+int msgpu_resonances_multi(int nmed, const msgpu_medium* const* meds, const double* doms, int* nz, msgpu_cplx* z,
+                           int nexpsup, int maxit, double toler, int mode);
+
 /* Resonances: Maehly-Aberth-Ehrlich on det M(k) (root_finder.c:522-577) with every
  * live start's f'/f evaluated in one batched launch per sweep.  z[0..*nz) holds the
  * starts on entry and the converged roots (sorted by real part) on exit.

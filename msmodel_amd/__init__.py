@@ -76,6 +76,12 @@ _SIG = {
                                                _dp]),
     "msgpu_resonances": (ctypes.c_int, [_mp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "msgpu_tracelog_multi": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_mp), ctypes.c_int, _ip, _dp, _dp,
+                                            ctypes.c_int, ctypes.c_double, _ip]),
+    "msgpu_dk_log_det_multi": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_mp), ctypes.c_int, _ip, _dp, _dp,
+                                              ctypes.c_int]),
+    "msgpu_resonances_multi": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_mp), _dp, _ip, _dp, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "init_scmodel": (ctypes.c_int, [ctypes.POINTER(ScModel), ctypes.c_int, ctypes.c_int, _dp]),
     "init_medium": (None, [_mp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ScModel), ctypes.c_int, ctypes.c_double]),
     "del_medium": (None, [_mp]),
@@ -322,3 +328,45 @@ def zgetrs(lu, ipiv, b):
     lib.zgetrs_(b"N", ctypes.byref(n), ctypes.byref(nrhs), _cp(lu), ctypes.byref(n), ip.ctypes.data_as(_ip), _cp(b),
                 ctypes.byref(n), ctypes.byref(info))
     return b, info.value
+
+
+# ------------------------------------------------------------------ batches over several media
+def _media(meds):
+    arr = (_mp * len(meds))()
+    for i, m in enumerate(meds):
+        arr[i] = ctypes.pointer(m.c)
+    return arr
+
+
+def tracelog_multi(meds, med_of, ks, mode=FP64, reeval_rel=1e-6):
+    """Trace-log of items (meds[med_of[i]], ks[i]); returns (out, flags)."""
+    ks = np.ascontiguousarray(ks, dtype=np.complex128)
+    mi = np.ascontiguousarray(med_of, dtype=np.int32)
+    assert mi.size == ks.size
+    out = np.zeros_like(ks)
+    flags = np.zeros(ks.size, dtype=np.int32)
+    _check(lib.msgpu_tracelog_multi(len(meds), _media(meds), ks.size, mi.ctypes.data_as(_ip), _cp(ks), _cp(out),
+                                    mode, reeval_rel, flags.ctypes.data_as(_ip)), "msgpu_tracelog_multi")
+    return out, flags
+
+
+def dk_log_det_multi(meds, med_of, ks, mode=FP64):
+    ks = np.ascontiguousarray(ks, dtype=np.complex128)
+    mi = np.ascontiguousarray(med_of, dtype=np.int32)
+    assert mi.size == ks.size
+    out = np.zeros_like(ks)
+    _check(lib.msgpu_dk_log_det_multi(len(meds), _media(meds), ks.size, mi.ctypes.data_as(_ip), _cp(ks), _cp(out),
+                                      mode), "msgpu_dk_log_det_multi")
+    return out
+
+
+def resonances_multi(meds, starts, doms, nexpsup=4, maxit=50, toler=1e-12, mode=FP64):
+    """starts[m], doms[m] = (xmin, xmax, ymin, ymax) per medium; returns ([roots of medium m], sweeps)."""
+    nz = np.array([len(s) for s in starts], dtype=np.int32)
+    z = np.ascontiguousarray(np.concatenate([np.asarray(s, dtype=np.complex128) for s in starts]))
+    dm = np.ascontiguousarray(np.asarray(doms, dtype=np.float64).reshape(-1))
+    assert dm.size == 4 * len(meds) and nz.size == len(meds)
+    offs = np.concatenate([[0], np.cumsum(nz)[:-1]])
+    sweeps = _check(lib.msgpu_resonances_multi(len(meds), _media(meds), _cp(dm), nz.ctypes.data_as(_ip), _cp(z),
+                                               nexpsup, maxit, toler, mode), "msgpu_resonances_multi")
+    return [z[o: o + c].copy() for o, c in zip(offs, nz)], sweeps

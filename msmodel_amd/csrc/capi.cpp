@@ -84,38 +84,205 @@ MSGPU_API int msgpu_profile_end(double* ms, double* flops, double* bytes, int* c
     return profile_end(ms, flops, bytes, count, ncat);
 }
 
-MSGPU_API int msgpu_tracelog_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode,
-                                   double reeval_rel, int* flags) {
-    const MediumView mv = view_of(med);
+}  // extern "C"
+
+namespace {
+
+std::vector<MediumView> views_of(int nmed, const msgpu_medium* const* meds) {
+    std::vector<MediumView> v;
+    for (int m = 0; m < nmed; m++) v.push_back(view_of(meds[m]));
+    return v;
+}
+
+// Trace-log of items (meds[midx[i]], ks[i]) in FP64, FP32 or MIXED.  MIXED: complex64 for
+// every item, then complex128 for the items whose N*Re(tracelog) lies below their own
+// medium's maximum + ln(reeval_rel) (|det| below reeval_rel of that medium's largest
+// |det| in the batch: near zeros of det M, where the complex64 factor loses the most
+// relative accuracy) and for non-finite complex64 results.  Returns #re-evaluated.
+int tracelog_any(const std::vector<MediumView>& mv, int nk, const int* midx, const msgpu_cplx* ks, msgpu_cplx* out,
+                 int mode, double reeval_rel, int* flags) {
     if (mode == MSGPU_FP64 || mode == MSGPU_FP32) {
-        if (flags) memset(flags, 0, sizeof(int) * (nk > 0 ? nk : 0));
-        return tracelog_points(mv, nk, as_cx(ks), as_cx(out), nullptr, mode == MSGPU_FP32 ? 1 : 0);
+        if (flags && nk > 0) memset(flags, 0, sizeof(int) * nk);
+        return tracelog_multi(mv, nk, midx, as_cx(ks), as_cx(out), nullptr, mode == MSGPU_FP32 ? 1 : 0);
     }
     if (mode != MSGPU_MIXED) { set_error("msgpu: unknown mode"); return -4; }
-    std::vector<double> stats(2 * (size_t)nk);
-    int rc = tracelog_points(mv, nk, as_cx(ks), as_cx(out), stats.data(), 1);
+    if (nk <= 0) return 0;
+    int rc = tracelog_multi(mv, nk, midx, as_cx(ks), as_cx(out), nullptr, 1);
     if (rc < 0) return rc;
-    // Cells to recompute in FP64: |det| below reeval_rel of the map maximum (near zeros of
-    // det M, where the complex64 factor loses the most relative accuracy) and cells whose
-    // complex64 result is not finite.
-    double best = -INFINITY;
-    for (int l = 0; l < nk; l++)
-        if (std::isfinite(out[l].real())) best = std::max(best, mv.n * out[l].real());
-    const double cut = reeval_rel > 0 ? best + std::log(reeval_rel) : -INFINITY;
-    std::vector<int> idx;
+    std::vector<double> best(mv.size(), -INFINITY);
     for (int l = 0; l < nk; l++) {
-        const bool f = (mv.n * out[l].real() < cut) || !std::isfinite(out[l].real());
+        const int m = midx ? midx[l] : 0;
+        if (std::isfinite(out[l].real())) best[m] = std::max(best[m], mv[m].n * out[l].real());
+    }
+    std::vector<int> idx, sub_m;
+    for (int l = 0; l < nk; l++) {
+        const int m = midx ? midx[l] : 0;
+        const double cut = reeval_rel > 0 ? best[m] + std::log(reeval_rel) : -INFINITY;
+        const bool f = (mv[m].n * out[l].real() < cut) || !std::isfinite(out[l].real());
         if (flags) flags[l] = f ? 1 : 0;
-        if (f) idx.push_back(l);
+        if (f) { idx.push_back(l); sub_m.push_back(m); }
     }
     if (!idx.empty()) {
         std::vector<msgpu_cplx> sub(idx.size()), res(idx.size());
         for (size_t i = 0; i < idx.size(); i++) sub[i] = ks[idx[i]];
-        rc = tracelog_points(mv, (int)idx.size(), as_cx(sub.data()), as_cx(res.data()), nullptr, 0);
+        rc = tracelog_multi(mv, (int)idx.size(), sub_m.data(), as_cx(sub.data()), as_cx(res.data()), nullptr, 0);
         if (rc < 0) return rc;
         for (size_t i = 0; i < idx.size(); i++) out[idx[i]] = res[i];
     }
     return (int)idx.size();
+}
+
+// Maehly-Aberth-Ehrlich on det M(k) for several media at once (root_finder.c:522-577).
+// Per medium: duplicate removal and the open-domain filter of the starts, the
+// exponential background fitted from 6*nexpsup Halton samples of f'/f
+// (root_finder.c:161-172), then simultaneous sweeps; every sweep evaluates f'/f for
+// the live roots of all media in one batched call per precision.  In MSGPU_MIXED a
+// medium's sweeps run on complex64 factors until all its steps are below
+// max(toler, 1e-5)|z|, then complex128 sweeps polish to toler.
+struct AberthState {
+    double x0, x1, y0, y1;
+    std::vector<cd> live, poly;
+    std::vector<int> flag;      // 0 live, 1 converged, -1 left the domain
+    int prec = 0;
+    bool done = false;
+};
+
+int resonances_any(const std::vector<MediumView>& mv, const double* doms, int* nz, msgpu_cplx* z, int nexpsup,
+                   int maxit, double toler, int mode) {
+    const int nmed = (int)mv.size();
+    std::vector<AberthState> S(nmed);
+    std::vector<int> offs(nmed, 0);
+    for (int m = 0, o = 0; m < nmed; m++) { offs[m] = o; o += nz[m]; }
+    for (int m = 0; m < nmed; m++) {
+        AberthState& A = S[m];
+        A.x0 = doms[4 * m]; A.x1 = doms[4 * m + 1]; A.y0 = doms[4 * m + 2]; A.y1 = doms[4 * m + 3];
+        std::vector<cd> zs;
+        for (int i = 0; i < nz[m]; i++) {
+            const cd c = z[offs[m] + i];
+            bool uniq = true;
+            for (const cd& o : zs)
+                if (std::abs(c - o) < toler * std::abs(c) || std::abs(c - o) < toler) { uniq = false; break; }
+            if (uniq) zs.push_back(c);
+        }
+        for (const cd& c : zs)
+            if (in_open(c, A.x0, A.x1, A.y0, A.y1)) A.live.push_back(c);
+        A.flag.assign(A.live.size(), 0);
+        A.poly.assign(std::max(nexpsup, 1), 0.0);
+        A.prec = (mode == MSGPU_FP64) ? 0 : 1;
+        A.done = A.live.size() <= 1;    // zero or one start: nothing to iterate (as before)
+        if (A.done) std::fill(A.flag.begin(), A.flag.end(), 1);
+    }
+    int rc;
+    if (nexpsup > 0) {      // background fit, all media in one FP64 call
+        const int ns = 6 * nexpsup;
+        std::vector<msgpu_cplx> zs, fs;
+        std::vector<int> ms;
+        for (int m = 0; m < nmed; m++) {
+            if (S[m].done) continue;
+            for (int i = 0; i < ns; i++) {
+                zs.push_back(cd(S[m].x0 + (S[m].x1 - S[m].x0) * halton(i + 1, 2),
+                                S[m].y0 + (S[m].y1 - S[m].y0) * halton(i + 1, 3)));
+                ms.push_back(m);
+            }
+        }
+        fs.resize(zs.size());
+        rc = dk_log_det_multi(mv, (int)zs.size(), ms.data(), as_cx(zs.data()), as_cx(fs.data()), 0);
+        if (rc < 0) return rc;
+        for (size_t base = 0; base < zs.size(); base += ns) {
+            const int m = ms[base];
+            std::vector<cd> a((size_t)ns * nexpsup), b(ns);
+            for (int i = 0; i < ns; i++) {
+                cd p = 1.0;
+                for (int j = 0; j < nexpsup; j++) { a[i + (size_t)j * ns] = p; p *= cd(zs[base + i]); }
+                b[i] = fs[base + i];
+            }
+            lstsq(ns, nexpsup, a, b);
+            for (int j = 0; j < nexpsup; j++) S[m].poly[j] = b[j];
+        }
+    }
+    const double coarse = std::max(toler, 1e-5);
+    int sweeps = 0;
+    for (int s = 0; s < maxit + 40; s++) {
+        // live roots of all media, split by the precision of their medium
+        std::vector<msgpu_cplx> zq[2], fq[2];
+        std::vector<int> mq[2], iq[2];
+        for (int m = 0; m < nmed; m++) {
+            if (S[m].done) continue;
+            for (size_t i = 0; i < S[m].live.size(); i++)
+                if (S[m].flag[i] == 0) {
+                    zq[S[m].prec].push_back(S[m].live[i]);
+                    mq[S[m].prec].push_back(m);
+                    iq[S[m].prec].push_back((int)i);
+                }
+        }
+        if (zq[0].empty() && zq[1].empty()) break;
+        sweeps++;
+        for (int p = 0; p < 2; p++) {
+            if (zq[p].empty()) continue;
+            fq[p].resize(zq[p].size());
+            rc = dk_log_det_multi(mv, (int)zq[p].size(), mq[p].data(), as_cx(zq[p].data()), as_cx(fq[p].data()), p);
+            if (rc < 0) return rc;
+        }
+        std::vector<std::vector<cd>> next(nmed);
+        std::vector<char> all_coarse(nmed, 1);
+        for (int m = 0; m < nmed; m++) next[m] = S[m].live;
+        for (int p = 0; p < 2; p++)
+            for (size_t q = 0; q < zq[p].size(); q++) {
+                const int m = mq[p][q], i = iq[p][q];
+                AberthState& A = S[m];
+                cd pv = 0.0;
+                for (int j = nexpsup - 1; j >= 0; j--) pv = pv * A.live[i] + A.poly[j];
+                cd invdz = cd(fq[p][q]) - pv;
+                for (size_t j = 0; j < A.live.size(); j++)
+                    if ((int)j != i) invdz -= 1.0 / (A.live[i] - A.live[j]);
+                const cd dz = 1.0 / invdz, zn = A.live[i] - dz;
+                const double tol_now = A.prec ? coarse : toler;
+                if (!in_open(zn, A.x0, A.x1, A.y0, A.y1)) A.flag[i] = -1;
+                else if (std::abs(dz) < tol_now * std::abs(A.live[i])) { if (!A.prec) A.flag[i] = 1; next[m][i] = zn; }
+                else { next[m][i] = zn; all_coarse[m] = 0; }
+            }
+        for (int m = 0; m < nmed; m++) {
+            if (S[m].done) continue;
+            S[m].live = next[m];
+            if (S[m].prec && all_coarse[m]) S[m].prec = 0;     // switch to the complex128 polish
+        }
+    }
+    for (int m = 0; m < nmed; m++) {
+        std::vector<cd> roots;
+        for (size_t i = 0; i < S[m].live.size(); i++)
+            if (S[m].flag[i] == 1) roots.push_back(S[m].live[i]);
+        std::sort(roots.begin(), roots.end(), [](const cd& a, const cd& b) { return a.real() < b.real(); });
+        nz[m] = (int)roots.size();
+        for (size_t i = 0; i < roots.size(); i++) z[offs[m] + i] = roots[i];
+    }
+    return sweeps;
+}
+
+}  // namespace
+
+extern "C" {
+
+MSGPU_API int msgpu_tracelog_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode,
+                                   double reeval_rel, int* flags) {
+    return tracelog_any(views_of(1, &med), nk, nullptr, ks, out, mode, reeval_rel, flags);
+}
+
+MSGPU_API int msgpu_tracelog_multi(int nmed, const msgpu_medium* const* meds, int nk, const int* med_of,
+                                   const msgpu_cplx* ks, msgpu_cplx* out, int mode, double reeval_rel, int* flags) {
+    if (nmed <= 0 || !meds) { set_error("msgpu: no medium given"); return -2; }
+    return tracelog_any(views_of(nmed, meds), nk, med_of, ks, out, mode, reeval_rel, flags);
+}
+
+MSGPU_API int msgpu_dk_log_det_multi(int nmed, const msgpu_medium* const* meds, int nk, const int* med_of,
+                                     const msgpu_cplx* ks, msgpu_cplx* out, int mode) {
+    if (nmed <= 0 || !meds) { set_error("msgpu: no medium given"); return -2; }
+    return dk_log_det_multi(views_of(nmed, meds), nk, med_of, as_cx(ks), as_cx(out), mode == MSGPU_FP64 ? 0 : 1);
+}
+
+MSGPU_API int msgpu_resonances_multi(int nmed, const msgpu_medium* const* meds, const double* doms, int* nz,
+                                     msgpu_cplx* z, int nexpsup, int maxit, double toler, int mode) {
+    if (nmed <= 0 || !meds) { set_error("msgpu: no medium given"); return -2; }
+    return resonances_any(views_of(nmed, meds), doms, nz, z, nexpsup, maxit, toler, mode);
 }
 
 MSGPU_API int msgpu_tracelog_stats(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out,
@@ -139,87 +306,11 @@ MSGPU_API int msgpu_wavefunction_grid(const msgpu_medium* med, const msgpu_iwave
                      as_cx(psi));
 }
 
-// Aberth-Ehrlich on det M(k) (root_finder.c:522-577): the exponential background is
-// fitted from 6*nexpsup Halton samples of f'/f (root_finder.c:161-172); then every
-// sweep evaluates f'/f for all live roots in one batched call and moves them
-// simultaneously.  In MSGPU_MIXED, sweeps run on complex64 factors until the step
-// falls below max(toler, 1e-5)|z|, then complex128 sweeps polish to toler.
+// Aberth-Ehrlich on det M(k) of one medium (root_finder.c:522-577); see resonances_any.
 MSGPU_API int msgpu_resonances(const msgpu_medium* med, double xmin, double xmax, double ymin, double ymax, int* nz,
                                msgpu_cplx* z, int nexpsup, int maxit, double toler, int mode) {
-    const MediumView mv = view_of(med);
-    int n0 = *nz;
-    std::vector<cd> zs;
-    for (int i = 0; i < n0; i++) {   // duplicate removal, then the open-domain filter
-        cd c = z[i];
-        bool uniq = true;
-        for (const cd& o : zs)
-            if (std::abs(c - o) < toler * std::abs(c) || std::abs(c - o) < toler) { uniq = false; break; }
-        if (uniq) zs.push_back(c);
-    }
-    std::vector<cd> live;
-    for (const cd& c : zs)
-        if (in_open(c, xmin, xmax, ymin, ymax)) live.push_back(c);
-    const int nr = (int)live.size();
-    if (nr <= 1) { *nz = nr; for (int i = 0; i < nr; i++) z[i] = live[i]; return 0; }
-    std::vector<cd> poly(std::max(nexpsup, 1), 0.0);
-    int rc;
-    if (nexpsup > 0) {
-        const int ns = 6 * nexpsup;
-        std::vector<msgpu_cplx> zsmp(ns), fs(ns);
-        for (int i = 0; i < ns; i++)
-            zsmp[i] = cd(xmin + (xmax - xmin) * halton(i + 1, 2), ymin + (ymax - ymin) * halton(i + 1, 3));
-        rc = dk_log_det_points(mv, ns, as_cx(zsmp.data()), as_cx(fs.data()), 0);
-        if (rc < 0) return rc;
-        std::vector<cd> a((size_t)ns * nexpsup), b(ns);
-        for (int i = 0; i < ns; i++) {
-            cd p = 1.0;
-            for (int j = 0; j < nexpsup; j++) { a[i + (size_t)j * ns] = p; p *= cd(zsmp[i]); }
-            b[i] = fs[i];
-        }
-        lstsq(ns, nexpsup, a, b);
-        for (int j = 0; j < nexpsup; j++) poly[j] = b[j];
-    }
-    std::vector<int> flag(nr, 0);
-    int prec = (mode == MSGPU_FP64) ? 0 : 1;
-    const double coarse = std::max(toler, 1e-5);
-    int sweeps = 0;
-    for (int s = 0; s < maxit + 40 && sweeps < maxit + 40; s++, sweeps++) {
-        std::vector<int> act;
-        for (int i = 0; i < nr; i++) if (flag[i] == 0) act.push_back(i);
-        if (act.empty()) break;
-        std::vector<msgpu_cplx> za(act.size()), fa(act.size());
-        for (size_t q = 0; q < act.size(); q++) za[q] = live[act[q]];
-        rc = dk_log_det_points(mv, (int)act.size(), as_cx(za.data()), as_cx(fa.data()), prec);
-        if (rc < 0) return rc;
-        const double tol_now = prec ? coarse : toler;
-        bool all_coarse = true;
-        std::vector<cd> next = live;
-        for (size_t q = 0; q < act.size(); q++) {
-            const int i = act[q];
-            cd pv = 0.0;
-            for (int j = nexpsup - 1; j >= 0; j--) pv = pv * live[i] + poly[j];
-            cd invdz = cd(fa[q]) - pv;
-            for (int j = 0; j < nr; j++) if (j != i) invdz -= 1.0 / (live[i] - live[j]);
-            const cd dz = 1.0 / invdz, zn = live[i] - dz;
-            if (!in_open(zn, xmin, xmax, ymin, ymax)) flag[i] = -1;
-            else if (std::abs(dz) < tol_now * std::abs(live[i])) { if (!prec) flag[i] = 1; next[i] = zn; }
-            else { next[i] = zn; all_coarse = false; }
-        }
-        live = next;
-        if (prec && all_coarse) prec = 0;       // switch to the complex128 polish
-        if (s >= maxit && prec == 0) {
-            bool busy = false;
-            for (int i = 0; i < nr; i++) if (flag[i] == 0) busy = true;
-            if (!busy) break;
-        }
-        if (s >= maxit + 39) break;
-    }
-    std::vector<cd> roots;
-    for (int i = 0; i < nr; i++) if (flag[i] == 1) roots.push_back(live[i]);
-    std::sort(roots.begin(), roots.end(), [](const cd& a, const cd& b) { return a.real() < b.real(); });
-    *nz = (int)roots.size();
-    for (size_t i = 0; i < roots.size(); i++) z[i] = roots[i];
-    return sweeps;
+    const double dom[4] = {xmin, xmax, ymin, ymax};
+    return resonances_any(views_of(1, &med), dom, nz, z, nexpsup, maxit, toler, mode);
 }
 
 // ------------------------------------------------------------------ drop-in: ms_matrix_util.h

@@ -38,6 +38,12 @@ int profile_end(double* ms, double* flops, double* bytes, int* count, int ncat);
 // stats (may be null) receives {max log|U_ii|, min log|U_ii|} per cell.
 int tracelog_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, double* stats,
                     int precision);
+// The same over several media: item i is (meds[midx[i]], ks[i]) (midx null: medium 0).
+// Items of different padded sizes are grouped; results come back in input order.
+int tracelog_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                   cx<double>* out, double* stats, int precision);
+int dk_log_det_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                     cx<double>* out, int precision);
 // tr(M^-1 dM/dk) for nk wave numbers.
 int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, int precision);
 // Amplitudes a = M(k)^-1 phi; refine: complex64 LU + FP64 refinement to tol.  Returns #steps or <0.

@@ -358,112 +358,209 @@ struct Lanes {   // nstreams() leased streams; lane 0 owns the inputs, the other
     }
 };
 
+// ------------------------------------------------------------------ multi-medium batches
+// A batch is a list of (medium, k) items.  Items are grouped by padded size np
+// (largest first); each group is cut into chunks (chunk_size) that run round-robin
+// on the leased streams.  Every matrix of a chunk has the same np but may belong to
+// another medium: the assembly, the diagonal and the final reductions read the
+// item's own positions, N, d and scattering model, the LU kernels only see np.
+struct Plan {
+    struct Chunk { int np, start, count; };
+    std::vector<int> order;            // position in the device item list -> input index
+    std::vector<Chunk> chunks;
+    size_t mat_elems = 0, piv_ints = 0, map_ints = 0;
+    int max_b = 0;
+};
+
+static Plan make_plan(const std::vector<MediumView>& meds, int nk, const int* midx, size_t elem, int mats_per_item,
+                      int ns) {
+    Plan P;
+    P.order.resize(nk);
+    for (int i = 0; i < nk; i++) P.order[i] = i;
+    auto np_of = [&](int i) { return meds[midx ? midx[i] : 0].np; };
+    std::stable_sort(P.order.begin(), P.order.end(), [&](int a, int b) { return np_of(a) > np_of(b); });
+    for (int g0 = 0; g0 < nk;) {
+        const int np = np_of(P.order[g0]);
+        int g1 = g0;
+        while (g1 < nk && np_of(P.order[g1]) == np) g1++;
+        const int cnt = g1 - g0;
+        const int bsz = chunk_size(cnt, max_batch(np, elem, mats_per_item * ns));
+        for (int s0 = g0; s0 < g1; s0 += bsz) {
+            const int B = std::min(bsz, g1 - s0);
+            P.chunks.push_back({np, s0, B});
+            P.mat_elems = std::max(P.mat_elems, (size_t)B * np * np);
+            P.piv_ints = std::max(P.piv_ints, (size_t)B * np);
+            P.map_ints = std::max(P.map_ints, (size_t)B * pmap_ints(np, NB));
+            P.max_b = std::max(P.max_b, B);
+        }
+        g0 = g1;
+    }
+    return P;
+}
+
+// Positions of every medium in one device buffer, and the device item list in plan order.
+struct DeviceItems {
+    DBuf<double> pos;
+    DBuf<MatItem> items;
+    std::vector<double> hpos;
+    std::vector<MatItem> hitems;
+    DeviceItems(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                const std::vector<int>& order, hipStream_t st) {
+        std::vector<size_t> off(meds.size());
+        size_t tot = 0;
+        for (size_t m = 0; m < meds.size(); m++) { off[m] = tot; tot += (size_t)meds[m].n * meds[m].d; }
+        hpos.resize(std::max<size_t>(tot, 1));
+        for (size_t m = 0; m < meds.size(); m++)
+            std::copy(meds[m].pos, meds[m].pos + (size_t)meds[m].n * meds[m].d, hpos.begin() + off[m]);
+        pos.alloc(hpos.size());
+        HIPCHK(hipMemcpyAsync(pos.p, hpos.data(), sizeof(double) * hpos.size(), hipMemcpyHostToDevice, st));
+        hitems.resize(nk);
+        for (int q = 0; q < nk; q++) {
+            const int i = order[q], m = midx ? midx[i] : 0;
+            MatItem& it = hitems[q];
+            it.pos = pos.p + off[m];
+            it.n = meds[m].n;
+            it.d = meds[m].d;
+            it.m = meds[m].m;
+            it.k = ks[i];
+            it.diag0 = it.diag1 = cx<double>{0.0, 0.0};
+        }
+        items.alloc(nk);
+        HIPCHK(hipMemcpyAsync(items.p, hitems.data(), sizeof(MatItem) * nk, hipMemcpyHostToDevice, st));
+        launch_item_diag(items.p, nk, st);
+    }
+};
+
 template <class T>
-static void tracelog_impl(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, double* stats) {
+static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                          cx<double>* out, double* stats) {
     Lanes S;
-    const int np = mv.np;
-    const long long ms = (long long)np * np;
-    DBuf<double> dpos((size_t)mv.n * mv.d);
-    DBuf<cx<double>> dks(nk), ddiag(nk), dout(nk);
+    const Plan P = make_plan(meds, nk, midx, sizeof(cx<T>), 1, S.ns);
+    DeviceItems D(meds, nk, midx, ks, P.order, S[0]);
+    DBuf<cx<double>> dout(nk);
     DBuf<double> dstats(2 * (size_t)nk);
-    HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * mv.n * mv.d, hipMemcpyHostToDevice, S[0]));
-    HIPCHK(hipMemcpyAsync(dks.p, ks, sizeof(cx<double>) * nk, hipMemcpyHostToDevice, S[0]));
-    launch_diag(dks.p, nk, mv.m, mv.d, 0, ddiag.p, nullptr, S[0]);
     S.fork();
-    const int bsz = chunk_size(nk, max_batch(np, sizeof(cx<T>), S.ns));
-    const int nlane = std::min(S.ns, (nk + bsz - 1) / bsz);
+    const int nlane = std::min<int>(S.ns, (int)P.chunks.size());
     std::vector<DBuf<cx<T>>> mats(nlane);
     std::vector<DBuf<int>> ipiv(nlane), info(nlane), pmap(nlane);
     for (int i = 0; i < nlane; i++) {
-        mats[i].alloc((size_t)bsz * ms);
-        ipiv[i].alloc((size_t)bsz * np);
-        info[i].alloc(bsz);
-        pmap[i].alloc((size_t)bsz * pmap_ints(np, NB));
+        mats[i].alloc(P.mat_elems);
+        ipiv[i].alloc(P.piv_ints);
+        info[i].alloc(P.max_b);
+        pmap[i].alloc(P.map_ints);
     }
     // stagger: the first chunk of lane i starts after lane i-1 has finished its first panel
     std::vector<hipEvent_t> stag(nlane, nullptr);
     for (int i = 0; i + 1 < nlane; i++) HIPCHK(hipEventCreateWithFlags(&stag[i], hipEventDisableTiming));
-    int c = 0;
-    for (int done = 0; done < nk; done += bsz, c++) {
-        const int B = std::min(bsz, nk - done), ln = c % nlane;
+    for (size_t c = 0; c < P.chunks.size(); c++) {
+        const Plan::Chunk& ch = P.chunks[c];
+        const int B = ch.count, np = ch.np, ln = (int)(c % nlane);
+        const long long ms = (long long)np * np;
         hipStream_t st = S[ln];
-        if (c > 0 && c < nlane) HIPCHK(hipStreamWaitEvent(st, stag[c - 1], 0));
-        timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st, [&] {
-            launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag.p + done, nullptr, 0, mats[ln].p, ms, B,
-                               st);
-        });
+        if (c > 0 && (int)c < nlane) HIPCHK(hipStreamWaitEvent(st, stag[c - 1], 0));
+        timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st,
+              [&] { launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st); });
         factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, st,
-                  (c + 1 < nlane) ? stag[c] : nullptr);
+                  ((int)c + 1 < nlane) ? stag[c] : nullptr);
         timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {
-            launch_tracelog<T>(mats[ln].p, mv.n, np, ms, ipiv[ln].p, dout.p + done, dstats.p + 2 * done, B, st);
+            launch_tracelog<T>(mats[ln].p, 0, D.items.p + ch.start, np, ms, ipiv[ln].p, dout.p + ch.start,
+                               dstats.p + 2 * (size_t)ch.start, B, st);
         });
     }
     S.join();
-    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, S[0]));
-    if (stats) HIPCHK(hipMemcpyAsync(stats, dstats.p, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, S[0]));
+    std::vector<cx<double>> hout(nk);
+    std::vector<double> hst(stats ? 2 * (size_t)nk : 0);
+    HIPCHK(hipMemcpyAsync(hout.data(), dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, S[0]));
+    if (stats) HIPCHK(hipMemcpyAsync(hst.data(), dstats.p, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, S[0]));
     S.l[0].sync();
     for (hipEvent_t e : stag)
         if (e) (void)hipEventDestroy(e);
+    for (int q = 0; q < nk; q++) {
+        out[P.order[q]] = hout[q];
+        if (stats) { stats[2 * P.order[q]] = hst[2 * q]; stats[2 * P.order[q] + 1] = hst[2 * q + 1]; }
+    }
+}
+
+static int check_media(const std::vector<MediumView>& meds, int nk, const int* midx) {
+    if (meds.empty()) { set_error("msgpu: no medium given"); return -2; }
+    for (const MediumView& mv : meds)
+        if (mv.d < 1 || mv.d > 4 || mv.n < 1 || !mv.pos) {
+            set_error("msgpu: the device path supports d = 1..4 and N >= 1 with positions filled");
+            return -3;
+        }
+    if (midx)
+        for (int i = 0; i < nk; i++)
+            if (midx[i] < 0 || midx[i] >= (int)meds.size()) { set_error("msgpu: medium index out of range"); return -2; }
+    return 0;
+}
+
+int tracelog_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                   cx<double>* out, double* stats, int precision) {
+    if (nk <= 0) return 0;
+    if (int rc = check_media(meds, nk, midx)) return rc;
+    return guarded([&]() -> int {
+        if (precision == 1) tracelog_impl<float>(meds, nk, midx, ks, out, stats);
+        else tracelog_impl<double>(meds, nk, midx, ks, out, stats);
+        return 0;
+    });
 }
 
 int tracelog_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, double* stats,
                     int precision) {
-    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
-    if (nk <= 0) return 0;
-    return guarded([&]() -> int {
-        if (precision == 1) tracelog_impl<float>(mv, nk, ks, out, stats);
-        else tracelog_impl<double>(mv, nk, ks, out, stats);
-        return 0;
-    });
+    return tracelog_multi(std::vector<MediumView>{mv}, nk, nullptr, ks, out, stats, precision);
 }
 
 // ------------------------------------------------------------------ batched d/dk log det
 template <class T>
-static void dk_impl(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out) {
+static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                    cx<double>* out) {
     Lanes S;
-    const int np = mv.np;
-    const long long ms = (long long)np * np;
-    DBuf<double> dpos((size_t)mv.n * mv.d);
-    DBuf<cx<double>> dks(nk), ddiag(nk), ddiag1(nk), dout(nk);
-    HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * mv.n * mv.d, hipMemcpyHostToDevice, S[0]));
-    HIPCHK(hipMemcpyAsync(dks.p, ks, sizeof(cx<double>) * nk, hipMemcpyHostToDevice, S[0]));
-    launch_diag(dks.p, nk, mv.m, mv.d, 0, ddiag.p, nullptr, S[0]);
-    launch_diag(dks.p, nk, mv.m, mv.d, 1, ddiag1.p, nullptr, S[0]);
+    const Plan P = make_plan(meds, nk, midx, sizeof(cx<T>), 2, S.ns);
+    DeviceItems D(meds, nk, midx, ks, P.order, S[0]);
+    DBuf<cx<double>> dout(nk);
     S.fork();
-    const int bsz = chunk_size(nk, max_batch(np, sizeof(cx<T>), 2 * S.ns));
-    const int nlane = std::min(S.ns, (nk + bsz - 1) / bsz);
+    const int nlane = std::min<int>(S.ns, (int)P.chunks.size());
     std::vector<DBuf<cx<T>>> mats(nlane), rhs(nlane);
     std::vector<DBuf<int>> ipiv(nlane), info(nlane), pmap(nlane);
     for (int i = 0; i < nlane; i++) {
-        mats[i].alloc((size_t)bsz * ms);
-        rhs[i].alloc((size_t)bsz * ms);
-        ipiv[i].alloc((size_t)bsz * np);
-        info[i].alloc(bsz);
-        pmap[i].alloc((size_t)bsz * pmap_ints(np, NB));
+        mats[i].alloc(P.mat_elems);
+        rhs[i].alloc(P.mat_elems);
+        ipiv[i].alloc(P.piv_ints);
+        info[i].alloc(P.max_b);
+        pmap[i].alloc(P.map_ints);
     }
-    int c = 0;
-    for (int done = 0; done < nk; done += bsz, c++) {
-        const int B = std::min(bsz, nk - done), ln = c % nlane;
+    for (size_t c = 0; c < P.chunks.size(); c++) {
+        const Plan::Chunk& ch = P.chunks[c];
+        const int B = ch.count, np = ch.np, ln = (int)(c % nlane);
+        const long long ms = (long long)np * np;
         hipStream_t st = S[ln];
-        launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag.p + done, nullptr, 0, mats[ln].p, ms, B, st);
-        launch_assemble<T>(dpos.p, mv.n, mv.d, np, dks.p + done, ddiag1.p + done, nullptr, 1, rhs[ln].p, ms, B, st);
+        const MatItem* it = D.items.p + ch.start;
+        launch_assemble_items<T>(it, np, 0, mats[ln].p, ms, B, st);
+        launch_assemble_items<T>(it, np, 1, rhs[ln].p, ms, B, st);
         factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, st);
         lu_solve<T>(mats[ln].p, rhs[ln].p, np, np, B, pmap[ln].p, ms, st);
-        launch_diag_sum<T>(rhs[ln].p, mv.n, np, ms, dout.p + done, B, st);
+        launch_diag_sum<T>(rhs[ln].p, 0, it, np, ms, dout.p + ch.start, B, st);
     }
     S.join();
-    HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, S[0]));
+    std::vector<cx<double>> hout(nk);
+    HIPCHK(hipMemcpyAsync(hout.data(), dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, S[0]));
     S.l[0].sync();
+    for (int q = 0; q < nk; q++) out[P.order[q]] = hout[q];
+}
+
+int dk_log_det_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                     cx<double>* out, int precision) {
+    if (nk <= 0) return 0;
+    if (int rc = check_media(meds, nk, midx)) return rc;
+    return guarded([&]() -> int {
+        if (precision == 1) dk_impl<float>(meds, nk, midx, ks, out);
+        else dk_impl<double>(meds, nk, midx, ks, out);
+        return 0;
+    });
 }
 
 int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, int precision) {
-    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
-    if (nk <= 0) return 0;
-    return guarded([&]() -> int {
-        if (precision == 1) dk_impl<float>(mv, nk, ks, out);
-        else dk_impl<double>(mv, nk, ks, out);
-        return 0;
-    });
+    return dk_log_det_multi(std::vector<MediumView>{mv}, nk, nullptr, ks, out, precision);
 }
 
 // ------------------------------------------------------------------ amplitude solve
@@ -638,7 +735,7 @@ int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, i
         }
         // factor without left interchanges, solve, then apply them for the LAPACK-format output
         factor<double>(m.p, np, 1, dpiv.p, dinfo.p, pmap.p, false, L.st);
-        if (tracelog) launch_tracelog<double>(m.p, n, np, (long long)np * np, dpiv.p, tl.p, nullptr, 1, L.st);
+        if (tracelog) launch_tracelog<double>(m.p, n, nullptr, np, (long long)np * np, dpiv.p, tl.p, nullptr, 1, L.st);
         if (nrhs > 0) lu_solve<double>(m.p, x.p, np, nrhs, 1, pmap.p, np, L.st);
         for (int k0 = NB; k0 < np; k0 += NB) launch_swap_left<double>(m.p, np, (long long)np * np, pmap.p, k0, NB, 1, L.st);
         HIPCHK(hipGetLastError());

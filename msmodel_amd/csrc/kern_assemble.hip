@@ -34,6 +34,20 @@ void launch_diag(const cx<double>* ks, int nk, ScModel m, int d, int kind, cx<do
     hipLaunchKernelGGL(diag_kernel, dim3((nk + 63) / 64), dim3(64), 0, st, ks, nk, m, d, kind, diag, izero);
 }
 
+// Both diagonals of every item, each with its own scattering model and dimension.
+__global__ void item_diag_kernel(MatItem* items, int cnt) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= cnt) return;
+    MatItem& it = items[l];
+    const cx<double> h{1e-4, 0.0};
+    it.diag0 = invf(it.m, it.d, it.k);
+    it.diag1 = (invf(it.m, it.d, it.k + h) - invf(it.m, it.d, it.k - h)) / (2.0 * h);
+}
+
+void launch_item_diag(MatItem* items, int cnt, hipStream_t st) {
+    if (cnt > 0) hipLaunchKernelGGL(item_diag_kernel, dim3((cnt + 63) / 64), dim3(64), 0, st, items, cnt);
+}
+
 // ------------------------------------------------------------------ assembly
 // One workgroup per pair of 32x32 tiles (I <= J) of one matrix: it evaluates the
 // Green function once per unordered pair and writes both (I,J) and, through an
@@ -46,8 +60,21 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
                                                        const cx<double>* __restrict__ ks,
                                                        const cx<double>* __restrict__ diag,
                                                        const cx<double>* __restrict__ izero, int kind,
-                                                       cx<T>* __restrict__ out, long long mstride) {
+                                                       cx<T>* __restrict__ out, long long mstride,
+                                                       const MatItem* __restrict__ items) {
     const int b = blockIdx.y;
+    cx<double> kd, dg;
+    if (items) {            // per-matrix medium (a batch over several media of one padded size)
+        const MatItem& it = items[b];
+        pos = it.pos;
+        n = it.n;
+        d = it.d;
+        kd = it.k;
+        dg = kind == 1 ? it.diag1 : it.diag0;
+    } else {
+        kd = ks[b];
+        dg = diag[b];
+    }
     const int nt = np / AT;
     // decode the upper-triangular tile pair index
     int t = blockIdx.x, I = 0;
@@ -62,9 +89,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
         for (int c = 0; c < 4; c++) sx[side][a][c] = (atom < n && c < d) ? pos[(size_t)atom * d + c] : 0.0;
     }
     __syncthreads();
-    const cx<double> kd = ks[b];
     const cx<T> k{(T)kd.x, (T)kd.y};
-    const cx<double> dg = diag[b];
     cx<double> iz{1.0, 0.0};
     if (kind == 2) iz = izero[b];
     cx<T>* M = out + (size_t)b * mstride;
@@ -113,8 +138,20 @@ void launch_assemble(const double* pos, int n, int d, int np, const cx<double>* 
     const int nt = np / AT;
     dim3 grid(nt * (nt + 1) / 2, nbatch);
     hipLaunchKernelGGL(assemble_kernel<T>, grid, dim3(256), 0, st, pos, n, d, np, ks, diag, izero, kind, out,
-                       mstride);
+                       mstride, (const MatItem*)nullptr);
 }
+
+template <class T>
+void launch_assemble_items(const MatItem* items, int np, int kind, cx<T>* out, long long mstride, int nbatch,
+                           hipStream_t st) {
+    const int nt = np / AT;
+    dim3 grid(nt * (nt + 1) / 2, nbatch);
+    hipLaunchKernelGGL(assemble_kernel<T>, grid, dim3(256), 0, st, (const double*)nullptr, 0, 0, np,
+                       (const cx<double>*)nullptr, (const cx<double>*)nullptr, (const cx<double>*)nullptr, kind, out,
+                       mstride, items);
+}
+template void launch_assemble_items<double>(const MatItem*, int, int, cx<double>*, long long, int, hipStream_t);
+template void launch_assemble_items<float>(const MatItem*, int, int, cx<float>*, long long, int, hipStream_t);
 template void launch_assemble<double>(const double*, int, int, int, const cx<double>*, const cx<double>*,
                                       const cx<double>*, int, cx<double>*, long long, int, hipStream_t);
 template void launch_assemble<float>(const double*, int, int, int, const cx<double>*, const cx<double>*,

@@ -514,12 +514,14 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(cx<T>* C, const cx<T>* A,
 // out[b] = (sum_i log U_ii + i pi [odd interchanges]) / n  (ms_matrix_util.c:185-202),
 // accumulated in FP64; stats[b] = {max_i log|U_ii|, min_i log|U_ii|}.
 template <class T>
-__global__ __launch_bounds__(256) void tracelog_kernel(const cx<T>* __restrict__ A, int n, int np,
+__global__ __launch_bounds__(256) void tracelog_kernel(const cx<T>* __restrict__ A, int n,
+                                                       const MatItem* __restrict__ items, int np,
                                                        long long mstride, const int* __restrict__ ipiv_all,
                                                        cx<double>* __restrict__ out, double* __restrict__ stats) {
     __shared__ double sre[256], sim[256], smax[256], smin[256];
     __shared__ int sodd[256];
     const int b = blockIdx.x, tid = threadIdx.x;
+    if (items) n = items[b].n;
     const cx<T>* M = A + (size_t)b * mstride;
     const int* ipiv = ipiv_all + (size_t)b * np;
     double re = 0.0, im = 0.0, mx = -INFINITY, mn = INFINITY;
@@ -619,10 +621,12 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(const cx<T>* __restrict
 
 // out[b] = sum_{i<n} X[b][i,i] in FP64.
 template <class T>
-__global__ __launch_bounds__(256) void diag_sum_kernel(const cx<T>* __restrict__ X, int n, int ld, long long sx,
+__global__ __launch_bounds__(256) void diag_sum_kernel(const cx<T>* __restrict__ X, int n,
+                                                       const MatItem* __restrict__ items, int ld, long long sx,
                                                        cx<double>* __restrict__ out) {
     __shared__ double sre[256], sim[256];
     const int b = blockIdx.x, tid = threadIdx.x;
+    if (items) n = items[b].n;
     double re = 0.0, im = 0.0;
     for (int i = tid; i < n; i += 256) {
         const cx<T> v = X[(size_t)b * sx + (size_t)i + (size_t)i * ld];
@@ -745,9 +749,10 @@ void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int
 }
 
 template <class T>
-void launch_tracelog(const cx<T>* A, int n, int np, long long mstride, const int* ipiv, cx<double>* out,
-                     double* stats, int nbatch, hipStream_t st) {
-    hipLaunchKernelGGL(tracelog_kernel<T>, dim3(nbatch), dim3(256), 0, st, A, n, np, mstride, ipiv, out, stats);
+void launch_tracelog(const cx<T>* A, int n, const MatItem* items, int np, long long mstride, const int* ipiv,
+                     cx<double>* out, double* stats, int nbatch, hipStream_t st) {
+    hipLaunchKernelGGL(tracelog_kernel<T>, dim3(nbatch), dim3(256), 0, st, A, n, items, np, mstride, ipiv, out,
+                       stats);
 }
 
 void launch_build_perm(const int* pmap, int np, int nb, int* perm, int nbatch, hipStream_t st) {
@@ -773,8 +778,9 @@ void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int 
 }
 
 template <class T>
-void launch_diag_sum(const cx<T>* X, int n, int ld, long long sx, cx<double>* out, int nbatch, hipStream_t st) {
-    hipLaunchKernelGGL(diag_sum_kernel<T>, dim3(nbatch), dim3(256), 0, st, X, n, ld, sx, out);
+void launch_diag_sum(const cx<T>* X, int n, const MatItem* items, int ld, long long sx, cx<double>* out, int nbatch,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(diag_sum_kernel<T>, dim3(nbatch), dim3(256), 0, st, X, n, items, ld, sx, out);
 }
 
 template <class T1, class T2>
@@ -821,13 +827,14 @@ void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st) {
     template void launch_swap_trsm<T>(cx<T>*, int, long long, const int*, int, int, int, int, hipStream_t);           \
     template void launch_gemm_sub<T>(cx<T>*, const cx<T>*, const cx<T>*, int, int, int, int, int, int, long long,      \
                                      long long, long long, int, hipStream_t);                                          \
-    template void launch_tracelog<T>(const cx<T>*, int, int, long long, const int*, cx<double>*, double*, int,        \
-                                     hipStream_t);                                                                     \
+    template void launch_tracelog<T>(const cx<T>*, int, const MatItem*, int, long long, const int*, cx<double>*,     \
+                                     double*, int, hipStream_t);                                                       \
     template void launch_gather_rows<T>(cx<T>*, const cx<T>*, const int*, int, int, long long, long long, int,       \
                                         hipStream_t);                                                                  \
     template void launch_trsm_block<T>(const cx<T>*, cx<T>*, int, int, int, int, long long, long long, int,           \
                                        hipStream_t);                                                                   \
-    template void launch_diag_sum<T>(const cx<T>*, int, int, long long, cx<double>*, int, hipStream_t);               \
+    template void launch_diag_sum<T>(const cx<T>*, int, const MatItem*, int, long long, cx<double>*, int,             \
+                                     hipStream_t);                                                                     \
     template void launch_axpy<T>(cx<double>*, const cx<T>*, int, hipStream_t);                                       \
     template void launch_apply_map<T>(cx<T>*, int, int, const int*, int, int, long long, int, hipStream_t);           \
     template void launch_swap_left<T>(cx<T>*, int, long long, const int*, int, int, int, hipStream_t);                \

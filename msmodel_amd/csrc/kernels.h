@@ -7,7 +7,23 @@
 
 namespace msg {
 
+// One matrix of a batch whose matrices may come from different media (same padded
+// size np): where its scatterers are, its wave number and its diagonal.
+struct MatItem {
+    const double* pos;     // device, row-major [n][d]
+    int n, d;
+    ScModel m;
+    cx<double> k;
+    cx<double> diag0;      // F(k)^-1                       (filled by launch_item_diag)
+    cx<double> diag1;      // (F^-1(k+h) - F^-1(k-h)) / 2h  (filled by launch_item_diag)
+};
+
 // kern_assemble.hip
+void launch_item_diag(MatItem* items, int cnt, hipStream_t st);
+// kind 0: M, kind 1: dM/dk, from items[0..nbatch)
+template <class T>
+void launch_assemble_items(const MatItem* items, int np, int kind, cx<T>* out, long long mstride, int nbatch,
+                           hipStream_t st);
 void launch_diag(const cx<double>* ks, int nk, ScModel m, int d, int kind, cx<double>* diag, cx<double>* izero,
                  hipStream_t st);
 template <class T>
@@ -29,9 +45,10 @@ void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int 
 template <class T>
 void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int k, int lda, int ldb, int ldc,
                      long long sC, long long sA, long long sB, int nbatch, hipStream_t st);
+// n of matrix b is items[b].n when items is given, else n
 template <class T>
-void launch_tracelog(const cx<T>* A, int n, int np, long long mstride, const int* ipiv, cx<double>* out,
-                     double* stats, int nbatch, hipStream_t st);
+void launch_tracelog(const cx<T>* A, int n, const MatItem* items, int np, long long mstride, const int* ipiv,
+                     cx<double>* out, double* stats, int nbatch, hipStream_t st);
 void launch_build_perm(const int* pmap, int np, int nb, int* perm, int nbatch, hipStream_t st);
 template <class T>
 void launch_gather_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, int ncol, long long sdst,
@@ -40,7 +57,8 @@ template <class T>
 void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int upper, long long slu, long long sb,
                        int nbatch, hipStream_t st);
 template <class T>
-void launch_diag_sum(const cx<T>* X, int n, int ld, long long sx, cx<double>* out, int nbatch, hipStream_t st);
+void launch_diag_sum(const cx<T>* X, int n, const MatItem* items, int ld, long long sx, cx<double>* out, int nbatch,
+                     hipStream_t st);
 template <class T1, class T2>
 void launch_convert(const cx<T1>* x, cx<T2>* y, long long count, hipStream_t st);
 void launch_residual(const cx<double>* M, const cx<double>* a, const cx<double>* phi, int np, cx<double>* r,
