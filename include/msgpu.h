@@ -178,6 +178,64 @@ int msgpu_resonances_multi(int nmed, const msgpu_medium* const* meds, const doub
 int msgpu_resonances(const msgpu_medium* med, double xmin, double xmax, double ymin, double ymax, int* nz,
                      msgpu_cplx* z, int nexpsup, int maxit, double toler, int mode);
 
+/* ------------------------------------------------------- 3. serving loop */
+/* No counterpart in the reference (msmain runs one task at a time).  A server admits
+ * log-det tiles, Foldy-Lax solves (+ psi grid) and resonance searches of mixed N and d
+ * as they arrive, by device footprint (16 N^2 bytes per complex128 matrix, 8 N^2 per
+ * complex64 one, times the matrices a request keeps live), keeps each admitted
+ * request's positions on the device, and batches the current stage of every admitted
+ * log-det and resonance request into shared multi-medium launches (continuous
+ * batching); solves run concurrently on their own thread.  Inputs are copied at
+ * submit; output buffers must stay valid until msgpu_server_wait returns.  A result
+ * does not depend on the other requests in flight. */
+enum { MSGPU_REQ_LOGDET = 0, MSGPU_REQ_SOLVE = 1, MSGPU_REQ_RESONANCE = 2 };
+typedef struct {
+    int kind;                   /* MSGPU_REQ_*                                               */
+    const msgpu_medium* med;    /* positions and model are copied at submit                  */
+    int mode;                   /* MSGPU_FP64 / MSGPU_MIXED / MSGPU_FP32                      */
+    /* LOGDET: out[l] = tracelog(M(ks[l])), l < nk, as msgpu_tracelog_batch            */
+    int nk;
+    const msgpu_cplx* ks;
+    double reeval_rel;
+    msgpu_cplx* out;
+    /* SOLVE: a[N] as msgpu_solve; psi[nx*ny] as msgpu_wavefunction_grid when nx*ny > 0 */
+    msgpu_cplx k;
+    int source;
+    double tol;
+    msgpu_cplx* a;
+    int nx, ny;
+    double xmin, xmax, ymin, ymax;
+    msgpu_cplx* psi;
+    /* RESONANCE: starts z[0..nz) in the open domain dom = {xmin, xmax, ymin, ymax}, as
+     * msgpu_resonances; the roots go to roots[0..*nroots)                             */
+    int nz;
+    const msgpu_cplx* z;
+    double dom[4];
+    int nexpsup, maxit;
+    double toler;
+    msgpu_cplx* roots;
+    int* nroots;
+} msgpu_request;
+typedef struct {
+    int status;                 /* 0 ok, < 0 error (message in msgpu_last_error after wait)  */
+    int count;                  /* LOGDET cells re-evaluated, SOLVE refinement steps,
+                                   RESONANCE sweeps                                          */
+    double queue_ms;            /* submit -> admitted                                        */
+    double total_ms;            /* submit -> done                                            */
+} msgpu_result;
+typedef struct msgpu_server msgpu_server;
+/* hbm_gb: admission budget in GiB; max_active: most requests admitted at once. */
+msgpu_server* msgpu_server_create(double hbm_gb, int max_active);
+/* Returns the request id (>= 0) or < 0 on a malformed request. */
+long long msgpu_server_submit(msgpu_server* s, const msgpu_request* r);
+/* Blocks until the request is done; returns its status.  Each id is waited for once. */
+int msgpu_server_wait(msgpu_server* s, long long id, msgpu_result* res);
+/* stats[0..nstats): steps, batched matrices, most requests in one step, peak admitted
+ * bytes, budget bytes. */
+int msgpu_server_stats(msgpu_server* s, double* stats, int nstats);
+/* Serves every submitted request, then stops the loops and frees the server. */
+void msgpu_server_destroy(msgpu_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,21 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 _mp = ctypes.POINTER(CMedium)
 
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("med", _mp), ("mode", ctypes.c_int),
+                ("nk", ctypes.c_int), ("ks", _dp), ("reeval_rel", ctypes.c_double), ("out", _dp),
+                ("k", Cplx), ("source", ctypes.c_int), ("tol", ctypes.c_double), ("a", _dp),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+                ("ymin", ctypes.c_double), ("ymax", ctypes.c_double), ("psi", _dp),
+                ("nz", ctypes.c_int), ("z", _dp), ("dom", ctypes.c_double * 4), ("nexpsup", ctypes.c_int),
+                ("maxit", ctypes.c_int), ("toler", ctypes.c_double), ("roots", _dp), ("nroots", _ip)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("count", ctypes.c_int), ("queue_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
 lib = ctypes.CDLL(LIB_PATH)
 _SIG = {
     "msgpu_init": (ctypes.c_int, []),
@@ -82,6 +97,11 @@ _SIG = {
                                               ctypes.c_int]),
     "msgpu_resonances_multi": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_mp), _dp, _ip, _dp, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "msgpu_server_create": (ctypes.c_void_p, [ctypes.c_double, ctypes.c_int]),
+    "msgpu_server_submit": (ctypes.c_longlong, [ctypes.c_void_p, ctypes.POINTER(Request)]),
+    "msgpu_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(Result)]),
+    "msgpu_server_stats": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int]),
+    "msgpu_server_destroy": (None, [ctypes.c_void_p]),
     "init_scmodel": (ctypes.c_int, [ctypes.POINTER(ScModel), ctypes.c_int, ctypes.c_int, _dp]),
     "init_medium": (None, [_mp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ScModel), ctypes.c_int, ctypes.c_double]),
     "del_medium": (None, [_mp]),
@@ -370,3 +390,78 @@ def resonances_multi(meds, starts, doms, nexpsup=4, maxit=50, toler=1e-12, mode=
     sweeps = _check(lib.msgpu_resonances_multi(len(meds), _media(meds), _cp(dm), nz.ctypes.data_as(_ip), _cp(z),
                                                nexpsup, maxit, toler, mode), "msgpu_resonances_multi")
     return [z[o: o + c].copy() for o, c in zip(offs, nz)], sweeps
+
+
+# ------------------------------------------------------------------ serving loop
+REQ_LOGDET, REQ_SOLVE, REQ_RESONANCE = 0, 1, 2
+
+
+class Server:
+    """msgpu_server_*: requests admitted by device footprint and served with continuous
+    batching.  submit_* return a handle; wait(handle) returns (outputs, result dict)."""
+
+    def __init__(self, hbm_gb=64.0, max_active=64):
+        init()
+        self.h = lib.msgpu_server_create(hbm_gb, max_active)
+        if not self.h:
+            raise MsgpuError("msgpu_server_create failed: %s" % lib.msgpu_last_error().decode())
+        self._live = {}
+
+    def _submit(self, req, keep, outputs):
+        rid = lib.msgpu_server_submit(self.h, ctypes.byref(req))
+        _check(rid, "msgpu_server_submit")
+        self._live[rid] = (req, keep, outputs)
+        return rid
+
+    def submit_logdet(self, med, ks, mode=FP64, reeval_rel=1e-6):
+        ks = np.ascontiguousarray(ks, dtype=np.complex128)
+        out = np.zeros_like(ks)
+        r = Request(kind=REQ_LOGDET, med=ctypes.pointer(med.c), mode=mode, nk=ks.size, ks=_cp(ks),
+                    reeval_rel=reeval_rel, out=_cp(out))
+        return self._submit(r, (ks,), {"tracelog": out})
+
+    def submit_solve(self, med, k, source=PLANE, mode=FP64, tol=0.0, grid=None):
+        a = np.zeros(med.n, dtype=np.complex128)
+        r = Request(kind=REQ_SOLVE, med=ctypes.pointer(med.c), mode=mode, k=_c(k), source=source, tol=tol, a=_cp(a))
+        outs = {"a": a}
+        if grid is not None:
+            nx, ny, x0, x1, y0, y1 = grid
+            psi = np.zeros(nx * ny, dtype=np.complex128)
+            r.nx, r.ny, r.xmin, r.xmax, r.ymin, r.ymax, r.psi = nx, ny, x0, x1, y0, y1, _cp(psi)
+            outs["psi"] = psi
+        return self._submit(r, (), outs)
+
+    def submit_resonance(self, med, starts, dom, nexpsup=4, maxit=50, toler=1e-12, mode=FP64):
+        z = np.ascontiguousarray(starts, dtype=np.complex128)
+        roots = np.zeros_like(z)
+        nr = np.zeros(1, dtype=np.int32)
+        r = Request(kind=REQ_RESONANCE, med=ctypes.pointer(med.c), mode=mode, nz=z.size, z=_cp(z),
+                    nexpsup=nexpsup, maxit=maxit, toler=toler, roots=_cp(roots), nroots=nr.ctypes.data_as(_ip))
+        r.dom[:] = list(dom)
+        return self._submit(r, (z,), {"roots": roots, "nroots": nr})
+
+    def wait(self, rid):
+        res = Result()
+        st = lib.msgpu_server_wait(self.h, rid, ctypes.byref(res))
+        _, _, outs = self._live.pop(rid)
+        _check(st, "server request %d" % rid)
+        if "roots" in outs:
+            outs = {"roots": outs["roots"][: int(outs["nroots"][0])].copy()}
+        return outs, {"count": res.count, "queue_ms": res.queue_ms, "total_ms": res.total_ms}
+
+    def stats(self):
+        v = np.zeros(5)
+        lib.msgpu_server_stats(self.h, _cp(v), 5)
+        return {"steps": int(v[0]), "batched_matrices": int(v[1]), "max_requests_per_step": int(v[2]),
+                "peak_admitted_bytes": v[3], "budget_bytes": v[4]}
+
+    def close(self):
+        if self.h:
+            lib.msgpu_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <vector>
 
+#include "aberth.h"
 #include "capi_internal.h"
 
 using namespace msg;
@@ -27,49 +28,6 @@ void die_on(int rc, const char* where) {
         fflush(stderr);
         exit(EXIT_FAILURE);
     }
-}
-
-// Small dense complex least squares min |A x - b| (Householder QR), A m x nc column-major.
-void lstsq(int m, int nc, std::vector<cd>& a, std::vector<cd>& b) {
-    for (int j = 0; j < nc; j++) {
-        double nrm = 0;
-        for (int i = j; i < m; i++) nrm += std::norm(a[i + j * m]);
-        nrm = std::sqrt(nrm);
-        if (nrm == 0) continue;
-        cd ajj = a[j + j * m];
-        cd alpha = (std::abs(ajj) > 0 ? -ajj / std::abs(ajj) : cd(-1)) * nrm;
-        std::vector<cd> v(m, 0.0);
-        for (int i = j; i < m; i++) v[i] = a[i + j * m];
-        v[j] -= alpha;
-        double vn = 0;
-        for (int i = j; i < m; i++) vn += std::norm(v[i]);
-        if (vn == 0) continue;
-        for (int c = j; c < nc; c++) {
-            cd s = 0;
-            for (int i = j; i < m; i++) s += std::conj(v[i]) * a[i + c * m];
-            s *= 2.0 / vn;
-            for (int i = j; i < m; i++) a[i + c * m] -= s * v[i];
-        }
-        cd s = 0;
-        for (int i = j; i < m; i++) s += std::conj(v[i]) * b[i];
-        s *= 2.0 / vn;
-        for (int i = j; i < m; i++) b[i] -= s * v[i];
-    }
-    for (int j = nc - 1; j >= 0; j--) {
-        cd s = b[j];
-        for (int c = j + 1; c < nc; c++) s -= a[j + c * m] * b[c];
-        b[j] = s / a[j + j * m];
-    }
-}
-
-double halton(int i, int base) {
-    double h = 0, f = 1;
-    while (i > 0) { f /= base; h += f * (i % base); i /= base; }
-    return h;
-}
-
-bool in_open(cd z, double x0, double x1, double y0, double y1) {
-    return x0 < z.real() && z.real() < x1 && y0 < z.imag() && z.imag() < y1;
 }
 
 }  // namespace
@@ -109,18 +67,21 @@ int tracelog_any(const std::vector<MediumView>& mv, int nk, const int* midx, con
     if (nk <= 0) return 0;
     int rc = tracelog_multi(mv, nk, midx, as_cx(ks), as_cx(out), nullptr, 1);
     if (rc < 0) return rc;
-    std::vector<double> best(mv.size(), -INFINITY);
-    for (int l = 0; l < nk; l++) {
-        const int m = midx ? midx[l] : 0;
-        if (std::isfinite(out[l].real())) best[m] = std::max(best[m], mv[m].n * out[l].real());
-    }
+    // per medium, the cells of that medium's items to recompute
+    std::vector<std::vector<int>> per(mv.size());
+    for (int l = 0; l < nk; l++) per[midx ? midx[l] : 0].push_back(l);
     std::vector<int> idx, sub_m;
-    for (int l = 0; l < nk; l++) {
-        const int m = midx ? midx[l] : 0;
-        const double cut = reeval_rel > 0 ? best[m] + std::log(reeval_rel) : -INFINITY;
-        const bool f = (mv[m].n * out[l].real() < cut) || !std::isfinite(out[l].real());
-        if (flags) flags[l] = f ? 1 : 0;
-        if (f) { idx.push_back(l); sub_m.push_back(m); }
+    if (flags) memset(flags, 0, sizeof(int) * nk);
+    for (size_t m = 0; m < mv.size(); m++) {
+        if (per[m].empty()) continue;
+        std::vector<msgpu_cplx> t(per[m].size());
+        for (size_t q = 0; q < per[m].size(); q++) t[q] = out[per[m][q]];
+        for (int q : reeval_cells(mv[m].n, (int)t.size(), t.data(), reeval_rel)) {
+            const int l = per[m][q];
+            if (flags) flags[l] = 1;
+            idx.push_back(l);
+            sub_m.push_back((int)m);
+        }
     }
     if (!idx.empty()) {
         std::vector<msgpu_cplx> sub(idx.size()), res(idx.size());
@@ -132,128 +93,67 @@ int tracelog_any(const std::vector<MediumView>& mv, int nk, const int* midx, con
     return (int)idx.size();
 }
 
-// Maehly-Aberth-Ehrlich on det M(k) for several media at once (root_finder.c:522-577).
-// Per medium: duplicate removal and the open-domain filter of the starts, the
-// exponential background fitted from 6*nexpsup Halton samples of f'/f
-// (root_finder.c:161-172), then simultaneous sweeps; every sweep evaluates f'/f for
-// the live roots of all media in one batched call per precision.  In MSGPU_MIXED a
-// medium's sweeps run on complex64 factors until all its steps are below
-// max(toler, 1e-5)|z|, then complex128 sweeps polish to toler.
-struct AberthState {
-    double x0, x1, y0, y1;
-    std::vector<cd> live, poly;
-    std::vector<int> flag;      // 0 live, 1 converged, -1 left the domain
-    int prec = 0;
-    bool done = false;
-};
-
+// Maehly-Aberth-Ehrlich on det M(k) for several media at once (root_finder.c:522-577,
+// stepped by AberthRun): the background fits of all media run as one batched FP64
+// call, then every sweep evaluates f'/f for the live roots of all media in one batched
+// call per precision.
 int resonances_any(const std::vector<MediumView>& mv, const double* doms, int* nz, msgpu_cplx* z, int nexpsup,
                    int maxit, double toler, int mode) {
     const int nmed = (int)mv.size();
-    std::vector<AberthState> S(nmed);
     std::vector<int> offs(nmed, 0);
     for (int m = 0, o = 0; m < nmed; m++) { offs[m] = o; o += nz[m]; }
-    for (int m = 0; m < nmed; m++) {
-        AberthState& A = S[m];
-        A.x0 = doms[4 * m]; A.x1 = doms[4 * m + 1]; A.y0 = doms[4 * m + 2]; A.y1 = doms[4 * m + 3];
-        std::vector<cd> zs;
-        for (int i = 0; i < nz[m]; i++) {
-            const cd c = z[offs[m] + i];
-            bool uniq = true;
-            for (const cd& o : zs)
-                if (std::abs(c - o) < toler * std::abs(c) || std::abs(c - o) < toler) { uniq = false; break; }
-            if (uniq) zs.push_back(c);
-        }
-        for (const cd& c : zs)
-            if (in_open(c, A.x0, A.x1, A.y0, A.y1)) A.live.push_back(c);
-        A.flag.assign(A.live.size(), 0);
-        A.poly.assign(std::max(nexpsup, 1), 0.0);
-        A.prec = (mode == MSGPU_FP64) ? 0 : 1;
-        A.done = A.live.size() <= 1;    // zero or one start: nothing to iterate (as before)
-        if (A.done) std::fill(A.flag.begin(), A.flag.end(), 1);
-    }
-    int rc;
-    if (nexpsup > 0) {      // background fit, all media in one FP64 call
-        const int ns = 6 * nexpsup;
+    std::vector<AberthRun> runs;
+    runs.reserve(nmed);
+    for (int m = 0; m < nmed; m++)
+        runs.emplace_back(z + offs[m], nz[m], doms[4 * m], doms[4 * m + 1], doms[4 * m + 2], doms[4 * m + 3],
+                          nexpsup, maxit, toler, mode != MSGPU_FP64);
+    // background fits
+    {
         std::vector<msgpu_cplx> zs, fs;
         std::vector<int> ms;
-        for (int m = 0; m < nmed; m++) {
-            if (S[m].done) continue;
-            for (int i = 0; i < ns; i++) {
-                zs.push_back(cd(S[m].x0 + (S[m].x1 - S[m].x0) * halton(i + 1, 2),
-                                S[m].y0 + (S[m].y1 - S[m].y0) * halton(i + 1, 3)));
-                ms.push_back(m);
+        for (int m = 0; m < nmed; m++)
+            for (const cd& c : runs[m].fit_points()) { zs.push_back(c); ms.push_back(m); }
+        if (!zs.empty()) {
+            fs.resize(zs.size());
+            const int rc = dk_log_det_multi(mv, (int)zs.size(), ms.data(), as_cx(zs.data()), as_cx(fs.data()), 0);
+            if (rc < 0) return rc;
+            for (size_t q = 0; q < zs.size();) {
+                const int m = ms[q];
+                std::vector<cd> f;
+                while (q < zs.size() && ms[q] == m) f.push_back(fs[q++]);
+                runs[m].take_fit(f);
             }
-        }
-        fs.resize(zs.size());
-        rc = dk_log_det_multi(mv, (int)zs.size(), ms.data(), as_cx(zs.data()), as_cx(fs.data()), 0);
-        if (rc < 0) return rc;
-        for (size_t base = 0; base < zs.size(); base += ns) {
-            const int m = ms[base];
-            std::vector<cd> a((size_t)ns * nexpsup), b(ns);
-            for (int i = 0; i < ns; i++) {
-                cd p = 1.0;
-                for (int j = 0; j < nexpsup; j++) { a[i + (size_t)j * ns] = p; p *= cd(zs[base + i]); }
-                b[i] = fs[base + i];
-            }
-            lstsq(ns, nexpsup, a, b);
-            for (int j = 0; j < nexpsup; j++) S[m].poly[j] = b[j];
         }
     }
-    const double coarse = std::max(toler, 1e-5);
     int sweeps = 0;
-    for (int s = 0; s < maxit + 40; s++) {
-        // live roots of all media, split by the precision of their medium
+    for (;;) {
         std::vector<msgpu_cplx> zq[2], fq[2];
-        std::vector<int> mq[2], iq[2];
+        std::vector<int> mq[2];
         for (int m = 0; m < nmed; m++) {
-            if (S[m].done) continue;
-            for (size_t i = 0; i < S[m].live.size(); i++)
-                if (S[m].flag[i] == 0) {
-                    zq[S[m].prec].push_back(S[m].live[i]);
-                    mq[S[m].prec].push_back(m);
-                    iq[S[m].prec].push_back((int)i);
-                }
+            int p = 0;
+            for (const cd& c : runs[m].sweep_points(&p)) { zq[p].push_back(c); mq[p].push_back(m); }
         }
         if (zq[0].empty() && zq[1].empty()) break;
-        sweeps++;
         for (int p = 0; p < 2; p++) {
             if (zq[p].empty()) continue;
             fq[p].resize(zq[p].size());
-            rc = dk_log_det_multi(mv, (int)zq[p].size(), mq[p].data(), as_cx(zq[p].data()), as_cx(fq[p].data()), p);
+            const int rc =
+                dk_log_det_multi(mv, (int)zq[p].size(), mq[p].data(), as_cx(zq[p].data()), as_cx(fq[p].data()), p);
             if (rc < 0) return rc;
         }
-        std::vector<std::vector<cd>> next(nmed);
-        std::vector<char> all_coarse(nmed, 1);
-        for (int m = 0; m < nmed; m++) next[m] = S[m].live;
         for (int p = 0; p < 2; p++)
-            for (size_t q = 0; q < zq[p].size(); q++) {
-                const int m = mq[p][q], i = iq[p][q];
-                AberthState& A = S[m];
-                cd pv = 0.0;
-                for (int j = nexpsup - 1; j >= 0; j--) pv = pv * A.live[i] + A.poly[j];
-                cd invdz = cd(fq[p][q]) - pv;
-                for (size_t j = 0; j < A.live.size(); j++)
-                    if ((int)j != i) invdz -= 1.0 / (A.live[i] - A.live[j]);
-                const cd dz = 1.0 / invdz, zn = A.live[i] - dz;
-                const double tol_now = A.prec ? coarse : toler;
-                if (!in_open(zn, A.x0, A.x1, A.y0, A.y1)) A.flag[i] = -1;
-                else if (std::abs(dz) < tol_now * std::abs(A.live[i])) { if (!A.prec) A.flag[i] = 1; next[m][i] = zn; }
-                else { next[m][i] = zn; all_coarse[m] = 0; }
+            for (size_t q = 0; q < zq[p].size();) {
+                const int m = mq[p][q];
+                std::vector<cd> f;
+                while (q < zq[p].size() && mq[p][q] == m) f.push_back(fq[p][q++]);
+                runs[m].take_sweep(f);
             }
-        for (int m = 0; m < nmed; m++) {
-            if (S[m].done) continue;
-            S[m].live = next[m];
-            if (S[m].prec && all_coarse[m]) S[m].prec = 0;     // switch to the complex128 polish
-        }
+        sweeps++;
     }
     for (int m = 0; m < nmed; m++) {
-        std::vector<cd> roots;
-        for (size_t i = 0; i < S[m].live.size(); i++)
-            if (S[m].flag[i] == 1) roots.push_back(S[m].live[i]);
-        std::sort(roots.begin(), roots.end(), [](const cd& a, const cd& b) { return a.real() < b.real(); });
-        nz[m] = (int)roots.size();
-        for (size_t i = 0; i < roots.size(); i++) z[offs[m] + i] = roots[i];
+        const std::vector<cd> r = runs[m].roots();
+        nz[m] = (int)r.size();
+        for (size_t i = 0; i < r.size(); i++) z[offs[m] + i] = r[i];
     }
     return sweeps;
 }

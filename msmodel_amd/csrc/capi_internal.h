@@ -1,6 +1,9 @@
 // capi_internal.h -- declarations shared by the host translation units of
 // libmsgpu.so (medium.cpp, engine.cpp, capi.cpp).
 #pragma once
+#include <algorithm>
+#include <cmath>
+#include <complex>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -11,6 +14,20 @@
 #define MSGPU_API __attribute__((visibility("default")))
 
 namespace msg {
+
+// Cells of a complex64 trace-log map of one medium (N = n) to recompute in complex128:
+// |det| below rel of the map's largest |det| (N Re t_l < max_l N Re t_l + ln rel; rel <= 0
+// disables this test) and non-finite cells.  Returns their indices in increasing order.
+inline std::vector<int> reeval_cells(int n, int nk, const std::complex<double>* t, double rel) {
+    double best = -INFINITY;
+    for (int l = 0; l < nk; l++)
+        if (std::isfinite(t[l].real())) best = std::max(best, n * t[l].real());
+    const double cut = rel > 0 ? best + std::log(rel) : -INFINITY;
+    std::vector<int> idx;
+    for (int l = 0; l < nk; l++)
+        if (n * t[l].real() < cut || !std::isfinite(t[l].real())) idx.push_back(l);
+    return idx;
+}
 
 // medium.cpp
 double ball_volume(int d);
@@ -23,6 +40,7 @@ struct MediumView {
     int d = 0, n = 0, np = 0;
     ScModel m{};
     const double* pos = nullptr;   // host, row-major [n][d]
+    const double* dpos = nullptr;  // device copy of pos when the caller keeps one resident
 };
 MediumView view_of(const msgpu_medium* med);
 inline int padded(int n) { return (n + 63) / 64 * 64; }
@@ -61,6 +79,9 @@ int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, i
             int* info);
 // Solve with an existing LAPACK-format LU from the host (zgetrs 'N').
 int lu_solve_host(int n, const cx<double>* lu, int lda, const int* ipiv, int nrhs, cx<double>* b, int ldb);
+// Device-resident copy of a medium's positions (released with release_positions).
+int upload_positions(const MediumView& mv, const double** dpos, size_t* token);
+void release_positions(const double* dpos, size_t token);
 // psi at arbitrary points (npt points, row-major [npt][d]).
 int wave_points(const MediumView& mv, cx<double> k, int source, const cx<double>* a, int npt, const double* pts,
                 cx<double>* psi);

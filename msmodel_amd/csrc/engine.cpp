@@ -257,6 +257,8 @@ template <class T>
 static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, hipStream_t st,
                    hipEvent_t after_first_panel = nullptr) {
     const long long ms = (long long)np * np;
+    // the panel kernel holds at most 8192 rows of a panel (1024 threads x 8 rows)
+    if (np > 8192) throw std::runtime_error("matrices above N = 8192 are not supported by the device LU");
     HIPCHK(hipMemsetAsync(info, 0, sizeof(int) * nbatch, st));
     const double es = sizeof(cx<T>);
     for (int k0 = 0; k0 < np; k0 += NB) {
@@ -407,18 +409,22 @@ struct DeviceItems {
     DeviceItems(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
                 const std::vector<int>& order, hipStream_t st) {
         std::vector<size_t> off(meds.size());
-        size_t tot = 0;
-        for (size_t m = 0; m < meds.size(); m++) { off[m] = tot; tot += (size_t)meds[m].n * meds[m].d; }
+        size_t tot = 0;      // media without a resident device copy are uploaded together
+        for (size_t m = 0; m < meds.size(); m++) {
+            off[m] = tot;
+            if (!meds[m].dpos) tot += (size_t)meds[m].n * meds[m].d;
+        }
         hpos.resize(std::max<size_t>(tot, 1));
         for (size_t m = 0; m < meds.size(); m++)
-            std::copy(meds[m].pos, meds[m].pos + (size_t)meds[m].n * meds[m].d, hpos.begin() + off[m]);
+            if (!meds[m].dpos)
+                std::copy(meds[m].pos, meds[m].pos + (size_t)meds[m].n * meds[m].d, hpos.begin() + off[m]);
         pos.alloc(hpos.size());
-        HIPCHK(hipMemcpyAsync(pos.p, hpos.data(), sizeof(double) * hpos.size(), hipMemcpyHostToDevice, st));
+        if (tot) HIPCHK(hipMemcpyAsync(pos.p, hpos.data(), sizeof(double) * tot, hipMemcpyHostToDevice, st));
         hitems.resize(nk);
         for (int q = 0; q < nk; q++) {
             const int i = order[q], m = midx ? midx[i] : 0;
             MatItem& it = hitems[q];
-            it.pos = pos.p + off[m];
+            it.pos = meds[m].dpos ? meds[m].dpos : pos.p + off[m];
             it.n = meds[m].n;
             it.d = meds[m].d;
             it.m = meds[m].m;
@@ -561,6 +567,20 @@ int dk_log_det_multi(const std::vector<MediumView>& meds, int nk, const int* mid
 
 int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, int precision) {
     return dk_log_det_multi(std::vector<MediumView>{mv}, nk, nullptr, ks, out, precision);
+}
+
+int upload_positions(const MediumView& mv, const double** dpos, size_t* token) {
+    return guarded([&]() -> int {
+        const size_t bytes = sizeof(double) * (size_t)mv.n * mv.d;
+        void* p = pool().get(bytes, *token);
+        HIPCHK(hipMemcpy(p, mv.pos, bytes, hipMemcpyHostToDevice));
+        *dpos = (const double*)p;
+        return 0;
+    });
+}
+
+void release_positions(const double* dpos, size_t token) {
+    if (dpos) pool().put((void*)dpos, token);
 }
 
 // ------------------------------------------------------------------ amplitude solve

@@ -1,0 +1,164 @@
+"""The serving loop (msgpu_server_*): mixed requests of different kinds, N and d.
+
+Each request is served alone (a server of its own, nothing else in flight) and
+then in a mix where all requests are in flight together under a small admission
+budget, so that they are admitted in stages and share batched steps.  Every
+result must be the same alone and in the mix; the comparison is exact because
+every matrix is computed on its own.  The alone results are also checked
+against the direct entry points (msgpu_tracelog_batch, msgpu_solve,
+msgpu_wavefunction_grid, msgpu_resonances), which the parity tests pin to the
+oracle.
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def G():
+    import msmodel_amd as G
+    G.init()
+    return G
+
+
+def make_requests(G):
+    """(kind, medium, args) for a reproducible mix."""
+    rng = np.random.default_rng(12)
+    reqs = []
+    specs = [("logdet", 3, 90, 1), ("solve", 2, 150, 2), ("resonance", 2, 24, 3), ("logdet", 2, 260, 4),
+             ("logdet", 3, 130, 5), ("solve", 3, 200, 6), ("resonance", 2, 40, 7), ("logdet", 3, 64, 8),
+             ("solve", 2, 70, 9), ("logdet", 4, 50, 10)]
+    for kind, d, n, seed in specs:
+        med = G.Medium(d, n, G.HARDSPHERE, (0.3 if d == 2 else 0.1,)).fill(seed)
+        if kind == "logdet":
+            ks = rng.uniform(2, 10, 24) + 1j * rng.uniform(-1, 0, 24)
+            mode = [G.FP64, G.MIXED, G.FP32][seed % 3]
+            reqs.append((kind, med, dict(ks=ks, mode=mode, reeval_rel=1e-2)))
+        elif kind == "solve":
+            mode = G.MIXED if seed % 2 else G.FP64
+            reqs.append((kind, med, dict(k=4.0 + 0.5 * seed, mode=mode, tol=1e-12,
+                                         grid=(12, 9, -3.0, 3.0, -2.0, 2.0))))
+        else:
+            dom = (1.0, 8.0, -1.0, 0.0)
+            st = dom[0] + (dom[1] - dom[0]) * rng.uniform(0.05, 0.95, 6) + \
+                1j * (dom[2] + (dom[3] - dom[2]) * rng.uniform(0.05, 0.95, 6))
+            reqs.append((kind, med, dict(starts=st, dom=dom, mode=G.MIXED if seed % 2 else G.FP64)))
+    return reqs
+
+
+def submit(srv, kind, med, a):
+    if kind == "logdet":
+        return srv.submit_logdet(med, a["ks"], a["mode"], a["reeval_rel"])
+    if kind == "solve":
+        return srv.submit_solve(med, a["k"], G_PLANE, a["mode"], a["tol"], a["grid"])
+    return srv.submit_resonance(med, a["starts"], a["dom"], 4, 50, 1e-12, a["mode"])
+
+
+G_PLANE = 0
+
+
+def footprint_bound(kind, med, a):
+    n2 = float(med.n) ** 2
+    if kind == "logdet":
+        return a["ks"].size * n2 * 16
+    if kind == "solve":
+        return n2 * 24
+    return 2 * a["starts"].size * n2 * 16
+
+
+def same(x, y):
+    assert x.keys() == y.keys()
+    for key in x:
+        assert np.array_equal(x[key], y[key]), key
+
+
+@pytest.fixture(scope="module")
+def alone(G):
+    reqs = make_requests(G)
+    outs = []
+    for kind, med, a in reqs:
+        srv = G.Server(hbm_gb=64.0)
+        print("alone", kind, med.c.d, med.n, flush=True)
+        outs.append(srv.wait(submit(srv, kind, med, a))[0])
+        srv.close()
+    return reqs, outs
+
+
+def test_alone_equals_direct_calls(G, alone):
+    reqs, outs = alone
+    for (kind, med, a), out in zip(reqs, outs):
+        if kind == "logdet":
+            want, _ = G.tracelog_batch(med, a["ks"], a["mode"], a["reeval_rel"])
+            assert np.array_equal(out["tracelog"], want)
+        elif kind == "solve":
+            aw, _ = G.solve(med, a["k"], G.PLANE, a["mode"], a["tol"])
+            assert np.array_equal(out["a"], aw)
+            psi = G.wavefunction_grid(med, a["k"], aw, *a["grid"])
+            assert np.array_equal(out["psi"], psi.ravel())
+        else:
+            rw, _ = G.resonances(med, a["starts"], a["dom"], 4, 50, 1e-12, a["mode"])
+            assert np.array_equal(out["roots"], rw)
+
+
+def test_mix_equals_alone(G, alone):
+    reqs, outs = alone
+    # a budget that holds only a few of these requests at once: staged admission
+    srv = G.Server(hbm_gb=0.02, max_active=6)
+    ids = [submit(srv, kind, med, a) for kind, med, a in reqs]
+    got = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    for (out, info), want in zip(got, outs):
+        same(out, want)
+        assert info["total_ms"] >= info["queue_ms"] >= 0
+    assert st["max_requests_per_step"] >= 2, st
+    # admitted bytes stay within the budget, except for one request larger than it alone
+    assert st["peak_admitted_bytes"] <= max(st["budget_bytes"], max(footprint_bound(*r) for r in reqs)), st
+
+
+def test_arrivals_while_busy(G, alone):
+    """Requests submitted from another thread while earlier ones are on the GPU."""
+    reqs, outs = alone
+    srv = G.Server(hbm_gb=8.0)
+    ids = [None] * len(reqs)
+
+    def late():
+        for j in range(1, len(reqs), 2):
+            time.sleep(0.002)
+            ids[j] = submit(srv, *reqs[j])
+
+    th = threading.Thread(target=late)
+    for j in range(0, len(reqs), 2):
+        ids[j] = submit(srv, *reqs[j])
+        if j == 0:
+            th.start()
+    th.join()
+    for j, i in enumerate(ids):
+        same(srv.wait(i)[0], outs[j])
+    srv.close()
+
+
+def test_logdet_vs_oracle(G):
+    med = G.Medium(3, 80, G.HARDSPHERE, (0.1,)).fill(21)
+    ks = np.array([3.0 - 0.2j, 6.5 - 0.05j, 9.0 + 0j])
+    srv = G.Server()
+    out = srv.wait(srv.submit_logdet(med, ks, G.FP64))[0]["tracelog"]
+    srv.close()
+    pos = O.fill_medium(3, 80, 21)
+    for k, t in zip(ks, out):
+        want = O.tracelog(O.build_matrix(pos, k, O.HARDSPHERE, (0.1,)))
+        assert abs(t.real - want.real) <= 1e-10 * max(1, abs(want.real))
+
+
+def test_malformed_request_rejected(G):
+    srv = G.Server()
+    med = G.Medium(3, 10, G.HARDSPHERE, (0.1,)).fill(1)
+    with pytest.raises(G.MsgpuError):
+        srv.submit_logdet(med, np.zeros(0, np.complex128))
+    srv.close()
