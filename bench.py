@@ -15,9 +15,12 @@ Paths timed on the same map and GPU:
   fp64 plain     one matrix per call in complex128 (the plain FP64 GPU path)
 value is the throughput of the fastest batched path whose error against the
 complex128 map meets |d tracelog| <= 1e-8 (value_path names it).
-Also: p50/p99 latency of a stream of resonance requests, the roofline of the
-dominant kernel from HIP events, and the reference msmain (gcc -O2, MKL) on a
-bounded sample of the same workload as CPU baseline.
+Also reported: a Poisson stream of mixed requests (log-det tiles, solves with a psi
+grid, resonance searches) through the serving loop with p50/p99 latency, a
+resonance search over 64 media of uneven N in one call, the N=1000 solve and
+1024^2 psi grid, the roofline of the dominant kernel from HIP events (HBM traffic
+from the recorded counter runs), and the reference msmain (gcc -O2, MKL) on a
+bounded sample of the same map as CPU baseline.
 """
 import argparse
 import json
@@ -43,6 +46,18 @@ def kgrid(nx, ny, xmin, xmax, ymin, ymax):
     dx, dy = (xmax - xmin) / (nx - 1), (ymax - ymin) / (ny - 1)
     l = np.arange(nx * ny)
     return (xmin + (l % nx) * dx) + 1j * (ymax - (l // nx) * dy)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch (read + write) of a kernel at this workload, from the counter
+    runs summarised in profiles/pmc_lu_map_n1000.md (FETCH_SIZE and WRITE_SIZE scaled by
+    a 1 GiB copy); None when no such run is recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        rec = json.load(open(path))["kernels"]["msg::" + kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return rec["read_bytes_per_launch"] + rec["write_bytes_per_launch"]
 
 
 def cpu_baseline(n, nx, ny):
@@ -86,22 +101,105 @@ def cpu_baseline(n, nx, ny):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
-def resonance_latencies(G, nreq):
-    """A stream of resonance requests: each is an Aberth search with 8 starts on its own medium
-    (d=2, N in [100, 400] log-uniform, unit-density disk, alpha=0.1), mixed precision."""
-    rng = np.random.default_rng(11)
-    lat = []
-    for r in range(nreq):
-        n = int(round(np.exp(rng.uniform(np.log(100), np.log(400)))))
-        med = G.Medium(2, n, G.HARDSPHERE, (0.1,)).fill(100 + r)
-        dom = (1.0, 12.0, -1.0, 0.0)
-        starts = dom[0] + (dom[1] - dom[0]) * rng.uniform(0.05, 0.95, 8) + \
-            1j * (dom[2] + (dom[3] - dom[2]) * rng.uniform(0.05, 0.95, 8))
-        t0 = time.perf_counter()
-        G.resonances(med, starts, dom, 4, 40, 1e-10, G.MIXED)
-        lat.append(time.perf_counter() - t0)
-    lat = np.array(lat) * 1e3
-    return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+def serving_stream(G, nreq, rate, nmin, nmax, seed=5):
+    """Poisson stream of mixed requests through the serving loop (msgpu_server_*).
+
+    Each request draws d in {2, 3}, N log-uniform in [nmin, nmax], a medium seed
+    (unit-density ball, hard spheres alpha=0.1) and one of: a 64-point log-det tile
+    (8x8 over Re k in [1,10] x Im k in [-1,0], mixed), a Foldy-Lax solve at real k in
+    [2,10] (mixed, refined to 1e-12) with a 256x256 psi grid over the medium, or a
+    32-start resonance search in (1,10) x (-1,0) (mixed, toler 1e-10).  Latency is
+    submit -> done as the server measures it."""
+    import threading
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(nreq):
+        d = int(rng.choice([2, 3]))
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        med = G.Medium(d, n, G.HARDSPHERE, (0.1,)).fill(int(rng.integers(1, 1 << 30)))
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            arg = kgrid(8, 8, 1.0, 10.0, -1.0, 0.0)
+        elif kind == 1:
+            r = float(np.abs(med.pos).max()) + 0.5
+            arg = (float(rng.uniform(2.0, 10.0)), (256, 256, -r, r, -r, r))
+        else:
+            arg = rng.uniform(1.2, 9.8, 32) + 1j * rng.uniform(-0.95, -0.05, 32)
+        specs.append((kind, med, arg, float(rng.exponential(1.0 / rate))))
+    srv = G.Server(hbm_gb=64.0, max_active=64)
+    ids = [None] * nreq
+
+    def submitter():
+        for j, (kind, med, arg, gap) in enumerate(specs):
+            time.sleep(gap)
+            if kind == 0:
+                ids[j] = srv.submit_logdet(med, arg, G.MIXED, 1e-6)
+            elif kind == 1:
+                ids[j] = srv.submit_solve(med, arg[0], G.PLANE, G.MIXED, 1e-12, arg[1])
+            else:
+                ids[j] = srv.submit_resonance(med, arg, (1.0, 10.0, -1.0, 0.0), 4, 30, 1e-10, G.MIXED)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    lat = {0: [], 1: [], 2: []}
+    for j, (kind, _, _, _) in enumerate(specs):
+        lat[kind].append(srv.wait(ids[j])[1]["total_ms"])
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    allv = np.array(sum(lat.values(), []))
+
+    def pct(v):
+        return {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99)), "n": len(v)} if v else None
+
+    return {"requests": nreq, "arrival_rate_per_s": rate, "N_range": [nmin, nmax], "wall_s": wall,
+            "throughput_req_per_s": nreq / wall, "latency_ms": pct(list(allv)),
+            "by_kind_ms": {"logdet_tile64": pct(lat[0]), "solve_psi256": pct(lat[1]), "resonance32": pct(lat[2])},
+            "steps": st["steps"], "batched_matrices": st["batched_matrices"],
+            "max_requests_per_step": st["max_requests_per_step"]}
+
+
+def uneven_search(G, nmed, per, nmin, nmax):
+    """Resonance search over media of uneven size in one call (msgpu_resonances_multi):
+    d=2, nmed media (seeds 100..) with N log-uniform in [nmin, nmax] in unit disks,
+    alpha=0.1, per starts each uniform in Re k in [1,40] x Im k in [-1,0], complex64
+    sweeps then complex128 polish to 1e-12."""
+    rng = np.random.default_rng(100)
+    meds, starts, doms = [], [], []
+    for s in range(100, 100 + nmed):
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        meds.append(G.Medium(2, n, G.HARDSPHERE, (0.1,)).fill(s, radius=1.0))
+        starts.append(rng.uniform(1.0, 40.0, per) + 1j * rng.uniform(-1.0, 0.0, per))
+        doms.append((1.0, 40.0, -1.0, 0.0))
+    t0 = time.perf_counter()
+    roots, sweeps = G.resonances_multi(meds, starts, doms, 4, 40, 1e-12, G.MIXED)
+    dt = time.perf_counter() - t0
+    return {"media": nmed, "starts": nmed * per, "N_range": [nmin, nmax], "sweeps": sweeps, "wall_s": dt,
+            "roots_found": int(sum(len(r) for r in roots)),
+            "sample": "%d media x %d starts (the full search runs 10000 starts)" % (nmed, per)}
+
+
+def wavefunction_case(G):
+    """Solve + psi grid: d=3, N=1000 in a ball of radius 5 (seed 2), alpha=0.1, plane
+    wave at real k=10, psi on 1024x1024 in the (x0, x1) plane over [-6, 6]^2."""
+    med = G.Medium(3, 1000, G.HARDSPHERE, (0.1,)).fill(2, radius=5.0)
+    G.solve(med, 10.0, G.PLANE, G.FP64)
+    t0 = time.perf_counter()
+    a64, _ = G.solve(med, 10.0, G.PLANE, G.FP64)
+    t_fp64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    a32, steps = G.solve(med, 10.0, G.PLANE, G.MIXED, 1e-12)
+    t_mixed = time.perf_counter() - t0
+    G.wavefunction_grid(med, 10.0, a64, 64, 64, -6, 6, -6, 6)
+    t0 = time.perf_counter()
+    psi = G.wavefunction_grid(med, 10.0, a64, 1024, 1024, -6, 6, -6, 6)
+    t_psi = time.perf_counter() - t0
+    return {"solve_fp64_ms": 1e3 * t_fp64, "solve_mixed_ms": 1e3 * t_mixed, "mixed_refinement_steps": steps,
+            "max_rel_da_mixed_vs_fp64": float(np.abs(a32 - a64).max() / np.abs(a64).max()),
+            "psi_grid_ms": 1e3 * t_psi, "green_evals_per_s": 1024 * 1024 * 1000 / t_psi,
+            "psi_finite": bool(np.isfinite(psi).all())}
 
 
 def main():
@@ -112,7 +210,13 @@ def main():
     ap.add_argument("--n", type=int, default=1000)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--plain", type=int, default=24, help="cells timed on the one-matrix-per-call FP64 path")
-    ap.add_argument("--requests", type=int, default=12, help="resonance requests for p50/p99")
+    ap.add_argument("--requests", type=int, default=48, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--rate", type=float, default=8.0, help="Poisson arrival rate of the serving stream (1/s)")
+    ap.add_argument("--serve-nmax", type=int, default=2000, help="largest N of the serving stream")
+    ap.add_argument("--uneven", type=int, default=64, help="media of the uneven resonance search (0: skip)")
+    ap.add_argument("--uneven-starts", type=int, default=4, help="starts per medium of the uneven search")
+    ap.add_argument("--no-wave", action="store_true", help="skip the solve + 1024^2 psi case")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shapes (no autotuning)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run this many device copies of 1 GiB (calibrates the HBM byte counters)")
@@ -137,6 +241,8 @@ def main():
     n, nk = args.n, args.grid * args.grid
     med = G.Medium(3, n, G.HARDSPHERE, (0.05,)).fill(1)
     ks = kgrid(args.grid, args.grid, 5.0, 30.0, -2.0, 0.0)
+    # set-up, untimed: the autotuner picks launch shapes for this N in both precisions
+    tuned = None if args.no_tune else {"fp64": G.autotune(n, G.FP64, 512), "fp32": G.autotune(n, G.FP32, 512)}
 
     def timed_path(mode):
         for _ in range(args.warmup):
@@ -165,7 +271,9 @@ def main():
     dtl = np.hypot(dre, dim)
     dtl_re = float(dre.max())
 
-    p50, p99 = resonance_latencies(G, args.requests) if args.requests > 0 else (None, None)
+    serving = serving_stream(G, args.requests, args.rate, 100, args.serve_nmax) if args.requests > 0 else None
+    uneven = uneven_search(G, args.uneven, args.uneven_starts, 200, 2000) if args.uneven > 0 else None
+    wave = None if args.no_wave else wavefunction_case(G)
 
     mixed_ok = float(dtl.max()) <= 1e-8
     if mixed_ok and mixed_rate > fp64_batched:
@@ -178,7 +286,7 @@ def main():
     g = prof["gemm"]
     achieved = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
+                "frac": achieved / peak, "traffic": measured_traffic(kname),
                 "kernel": kname + " (LU trailing update, MFMA 16x16x4)",
                 "launches": g["launches"], "avg_ms": g["ms"] / max(1, g["launches"])}
     g32 = prof32["gemm"]
@@ -201,8 +309,12 @@ def main():
         "mixed_reevaluated_cells": int(flags.sum()), "mixed_reeval_rel": 1e-6,
         "mixed_max_abs_dtracelog": float(dtl.max()), "mixed_median_abs_dtracelog": float(np.median(dtl)),
         "mixed_max_abs_dtracelog_re": dtl_re, "roofline_c64_gemm": roofline_c64,
-        "resonance_latency_ms": {"p50": p50, "p99": p99, "requests": args.requests},
+        "resonance_latency_ms": serving["by_kind_ms"]["resonance32"] if serving else None,
+        "serving": serving,
+        "uneven_resonance_search": uneven,
+        "solve_wavefunction_n1000": wave,
         "kernel_time_fraction": breakdown,
+        "autotune": tuned,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -124,6 +124,17 @@ const char* msgpu_last_error(void);
 int msgpu_profile_begin(void);
 int msgpu_profile_end(double* ms, double* flops, double* bytes, int* count, int ncat);
 
+/* Autotuner (no counterpart in the reference): times the batched trace-log of nitems
+ * matrices of size N (synthetic d=3 unit-density medium) for each storage order of the
+ * factorisation, swap + TRSM tile shape, chunk size and stream count, and keeps the fastest for N's padded size and
+ * the mode's precision (complex128 for MSGPU_FP64, complex64 otherwise); later calls
+ * use it.  report (may be NULL) = {swap variant, chunk, streams, best ms, row-major}.
+ * msgpu_tune_save/_load write/read the table as text; MSGPU_TUNE_FILE is loaded at
+ * initialisation.  _load returns the number of entries read. */
+int msgpu_autotune(int n, int mode, int nitems, double* report);
+int msgpu_tune_save(const char* path);
+int msgpu_tune_load(const char* path);
+
 /* Trace-log of M(k) for nk wave numbers of one medium: out[l] = tracelog_lu(M(ks[l]))
  * as in run_tracelog_omp (ms_core.c:122-127).  mode = MSGPU_FP64/MIXED/FP32.
  * For MSGPU_MIXED, cells with N*Re(tracelog) < max_l N*Re(tracelog) + ln(reeval_rel)

@@ -97,6 +97,9 @@ _SIG = {
                                               ctypes.c_int]),
     "msgpu_resonances_multi": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_mp), _dp, _ip, _dp, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "msgpu_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    "msgpu_tune_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "msgpu_tune_load": (ctypes.c_int, [ctypes.c_char_p]),
     "msgpu_server_create": (ctypes.c_void_p, [ctypes.c_double, ctypes.c_int]),
     "msgpu_server_submit": (ctypes.c_longlong, [ctypes.c_void_p, ctypes.POINTER(Request)]),
     "msgpu_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(Result)]),
@@ -348,6 +351,23 @@ def zgetrs(lu, ipiv, b):
     lib.zgetrs_(b"N", ctypes.byref(n), ctypes.byref(nrhs), _cp(lu), ctypes.byref(n), ip.ctypes.data_as(_ip), _cp(b),
                 ctypes.byref(n), ctypes.byref(info))
     return b, info.value
+
+
+# ------------------------------------------------------------------ autotuner
+def autotune(n, mode=FP64, nitems=512):
+    """Pick the fastest swap + TRSM shape, chunk size and stream count for size n."""
+    rep = np.zeros(5)
+    _check(lib.msgpu_autotune(n, mode, nitems, _cp(rep)), "msgpu_autotune")
+    return {"swap_variant": int(rep[0]), "chunk": int(rep[1]), "streams": int(rep[2]), "best_ms": float(rep[3]),
+            "row_major": int(rep[4])}
+
+
+def tune_save(path):
+    return _check(lib.msgpu_tune_save(path.encode()), "msgpu_tune_save")
+
+
+def tune_load(path):
+    return _check(lib.msgpu_tune_load(path.encode()), "msgpu_tune_load")
 
 
 # ------------------------------------------------------------------ batches over several media

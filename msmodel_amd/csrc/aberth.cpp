@@ -55,7 +55,7 @@ double halton(int i, int base) {
 AberthRun::AberthRun(const cd* starts, int nstart, double xmin, double xmax, double ymin, double ymax, int nexpsup,
                      int maxit, double toler, bool mixed)
     : x0_(xmin), x1_(xmax), y0_(ymin), y1_(ymax), toler_(toler), nexpsup_(std::max(nexpsup, 0)), maxit_(maxit),
-      prec_(mixed ? 1 : 0), fitted_(nexpsup <= 0) {
+      prec_(mixed ? 1 : 0), mixed_(mixed), fitted_(nexpsup <= 0) {
     std::vector<cd> zs;
     for (int i = 0; i < nstart; i++) {
         const cd c = starts[i];
@@ -97,9 +97,13 @@ void AberthRun::take_fit(const std::vector<cd>& f) {
     fitted_ = true;
 }
 
+// The reference stops after maxit sweeps (root_finder.c:546); a mixed-precision run
+// may add up to kPolish complex128 sweeps after its complex64 phase.
+static constexpr int kPolish = 10;
+
 bool AberthRun::finished() const {
     if (!fitted_) return false;
-    if (sweeps_ >= maxit_ + 40) return true;
+    if (sweeps_ >= maxit_ + (mixed_ ? kPolish : 0)) return true;
     for (int f : flag_)
         if (f == 0) return false;
     return true;
@@ -114,13 +118,17 @@ std::vector<cd> AberthRun::sweep_points(int* precision) const {
     return z;
 }
 
+// One sweep as root_finder.c:546-565: f'/f of every live root was evaluated at its
+// position before the sweep (a root's own evaluation never sees its update); the
+// repulsion terms then use the positions already updated in this sweep for the roots
+// before it (Gauss-Seidel order).  A converged root keeps its previous position.
 void AberthRun::take_sweep(const std::vector<cd>& f) {
     act_.clear();
     for (size_t i = 0; i < live_.size(); i++)
         if (flag_[i] == 0) act_.push_back((int)i);
     const double tol_now = prec_ ? std::max(toler_, 1e-5) : toler_;
     bool all_coarse = true;
-    std::vector<cd> next = live_;
+    std::vector<char> coarse(live_.size(), 0);
     for (size_t q = 0; q < act_.size(); q++) {
         const int i = act_[q];
         cd pv = 0.0;
@@ -129,13 +137,24 @@ void AberthRun::take_sweep(const std::vector<cd>& f) {
         for (size_t j = 0; j < live_.size(); j++)
             if ((int)j != i) invdz -= 1.0 / (live_[i] - live_[j]);
         const cd dz = 1.0 / invdz, zn = live_[i] - dz;
-        if (!in_open(zn)) flag_[i] = -1;
-        else if (std::abs(dz) < tol_now * std::abs(live_[i])) { if (!prec_) flag_[i] = 1; next[i] = zn; }
-        else { next[i] = zn; all_coarse = false; }
+        if (!in_open(zn)) {
+            flag_[i] = -1;
+        } else if (std::abs(dz) < tol_now * std::abs(live_[i])) {
+            if (!prec_) flag_[i] = 1;
+            else { live_[i] = zn; coarse[i] = 1; }   // complex64 phase: keep moving until the polish
+        } else {
+            live_[i] = zn;
+            all_coarse = false;
+        }
     }
-    live_ = next;
-    if (prec_ && all_coarse) prec_ = 0;     // switch to the complex128 polish
     sweeps_++;
+    // switch to the complex128 polish once every live root is coarse-converged, or when
+    // the complex64 phase has used its maxit sweeps (roots still far off are dropped)
+    if (prec_ && (all_coarse || sweeps_ >= maxit_)) {
+        for (int i : act_)
+            if (flag_[i] == 0 && !coarse[i]) flag_[i] = -1;
+        prec_ = 0;
+    }
 }
 
 std::vector<cd> AberthRun::roots() const {

@@ -7,10 +7,11 @@
 // background f'/f ~ poly(z) is fitted by least squares to 6*nexpsup Halton samples of
 // the domain (root_finder.c:161-172, Halton index i+1); every sweep moves each live
 // root by dz = 1/(f'/f - poly(z) - sum_{j != i} 1/(z_i - z_j)); a root converges when
-// |dz| < toler |z| and is abandoned when its step leaves the domain.  Sweeps are
-// simultaneous (every root moves from the previous sweep's positions).
+// |dz| < toler |z| (keeping its position) and is abandoned when its step leaves the
+// domain; at most maxit sweeps, roots not converged by then are dropped.
 // With mixed precision the sweeps use complex64 factors until every live step is
-// below max(toler, 1e-5)|z|, then complex128 sweeps polish to toler.
+// below max(toler, 1e-5)|z| (at most maxit sweeps), then up to 10 complex128 sweeps
+// polish to toler.
 #pragma once
 #include <complex>
 #include <vector>
@@ -41,7 +42,7 @@ private:
     std::vector<int> flag_;     // 0 live, 1 converged, -1 left the domain
     std::vector<int> act_;      // indices evaluated by the current sweep
     int prec_, sweeps_ = 0;
-    bool fitted_;
+    bool mixed_, fitted_;
     bool in_open(const cd& z) const { return x0_ < z.real() && z.real() < x1_ && y0_ < z.imag() && z.imag() < y1_; }
 };
 

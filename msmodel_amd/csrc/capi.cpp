@@ -36,6 +36,12 @@ extern "C" {
 
 // ------------------------------------------------------------------ batched API
 MSGPU_API int msgpu_init(void) { return engine_init(); }
+MSGPU_API int msgpu_autotune(int n, int mode, int nitems, double* report) {
+    if (engine_init() != 0) return -1;
+    return autotune(padded(n), mode == MSGPU_FP64 ? 0 : 1, nitems, report);
+}
+MSGPU_API int msgpu_tune_save(const char* path) { return save_tuning(path); }
+MSGPU_API int msgpu_tune_load(const char* path) { return load_tuning(path); }
 MSGPU_API const char* msgpu_last_error(void) { return last_error(); }
 MSGPU_API int msgpu_profile_begin(void) { return profile_begin(); }
 MSGPU_API int msgpu_profile_end(double* ms, double* flops, double* bytes, int* count, int ncat) {

@@ -3,6 +3,8 @@
 // pipelines built from the kernels of kern_*.hip.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -150,6 +152,7 @@ int engine_init() {
             return;
         }
         g_init_rc = 0;
+        if (const char* f = getenv("MSGPU_TUNE_FILE")) (void)load_tuning(f);
     });
     if (g_init_rc != 0) set_error("msgpu: no usable HIP device (hipGetDeviceCount/hipSetDevice failed)");
     return g_init_rc;
@@ -249,25 +252,51 @@ static int max_batch(int np, size_t elem, int mats_per_item) {
     return (int)b;
 }
 
+// ------------------------------------------------------------------ tuning table
+// Per (padded size, precision): the swap + TRSM tile shape, the chunk size, the
+// stream count and the storage order of the trace-log factorisation (rm: 1 row-major,
+// 0 column-major).  Filled by autotune() or load_tuning(); an entry of 0 keeps the
+// built-in choice (variant 0, MSGPU_CHUNK, MSGPU_STREAMS, column-major).
+struct Tune {
+    int swap = 0, chunk = 0, streams = 0, rm = 0;
+};
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, Tune> g_tune;
+
+static Tune tune_for(int np, int prec) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find({np, prec});
+    return it == g_tune.end() ? Tune{} : it->second;
+}
+template <class T> static int prec_of() { return sizeof(T) == 8 ? 0 : 1; }
+
 // ------------------------------------------------------------------ LU core
+// rm: the matrices are stored row-major.  The trailing update is the same GEMM call in
+// both orders: read column-major, the row-major blocks L21, U12 and A22 are U12^T,
+// L21^T and A22^T at the addresses of the column-major L21, U12 and A22, and
+// A22^T -= U12^T L21^T is the transposed update.  The pipelines of symmetric M(k)
+// use row-major storage, where the row interchanges move contiguous rows.
+// left: apply the interchanges to the columns left of the panel too (the full
+// P^T M = L U that the solves need); the trace-log needs only U's diagonal.
 // after_first_panel (optional) is recorded once the first panel is queued, so that a
 // concurrent stream can start its own factorization one panel phase later and the
 // latency-bound panels of one stream run under the GEMMs of the other.
 template <class T>
-static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, hipStream_t st,
-                   hipEvent_t after_first_panel = nullptr) {
+static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, bool rm,
+                   hipStream_t st, hipEvent_t after_first_panel = nullptr) {
     const long long ms = (long long)np * np;
     // the panel kernel holds at most 8192 rows of a panel (1024 threads x 8 rows)
     if (np > 8192) throw std::runtime_error("matrices above N = 8192 are not supported by the device LU");
     HIPCHK(hipMemsetAsync(info, 0, sizeof(int) * nbatch, st));
     const double es = sizeof(cx<T>);
+    const int swapv = tune_for(np, prec_of<T>()).swap;
     for (int k0 = 0; k0 < np; k0 += NB) {
         const int kend = k0 + NB, rest = np - kend;
         timed(0, 8.0 * nbatch * ((double)(np - k0) * NB * NB / 2), es * nbatch * 2.0 * (np - k0) * NB, st,
-              [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st); });
+              [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st, rm); });
         if (k0 == 0 && after_first_panel) HIPCHK(hipEventRecord(after_first_panel, st));
         timed(1, 8.0 * nbatch * (double)NB * NB / 2 * rest, es * nbatch * 2.0 * NB * rest, st,
-              [&] { launch_swap_trsm<T>(mats, np, ms, pmap, k0, NB, left ? 1 : 0, nbatch, st); });
+              [&] { launch_swap_trsm<T>(mats, np, ms, pmap, k0, NB, left ? 1 : 0, nbatch, st, swapv, rm); });
         if (kend < np)
             timed(2, 8.0 * nbatch * (double)rest * rest * NB, es * nbatch * (2.0 * rest * rest + 2.0 * rest * NB), st,
                   [&] {
@@ -302,6 +331,30 @@ static void lu_solve(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, co
     HIPCHK(hipGetLastError());
 }
 
+// X <- V with M V' = X solved from the row-major factor (left interchanges applied)
+// of a symmetric M, read column-major as S = M^T = U^T L^T P^T: first U^T W = X
+// (non-unit lower), then L^T V = W (unit upper).  The solution of M Y = X is then
+// Y = P V, i.e. Y[perm[i]] = V[i] with perm from launch_build_perm.
+template <class T>
+static void lu_solve_rm(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, long long sx, hipStream_t st) {
+    const long long ms = (long long)np * np;
+    const int nq = np / NB;
+    for (int q = 0; q < nq; q++) {
+        const int k0 = q * NB;
+        launch_trsm_block<T>(lu, X, np, ncol, k0, 0, ms, sx, nbatch, st, true);
+        if (k0 + NB < np)
+            launch_gemm_sub<T>(X + k0 + NB, lu + (k0 + NB) + (size_t)k0 * np, X + k0, np - k0 - NB, ncol, NB, np, np,
+                               np, sx, ms, sx, nbatch, st);
+    }
+    for (int q = nq - 1; q >= 0; q--) {
+        const int k0 = q * NB;
+        launch_trsm_block<T>(lu, X, np, ncol, k0, 1, ms, sx, nbatch, st, true);
+        if (k0 > 0)
+            launch_gemm_sub<T>(X, lu + (size_t)k0 * np, X + k0, k0, ncol, NB, np, np, np, sx, ms, sx, nbatch, st);
+    }
+    HIPCHK(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ batched trace-log
 // Chunking of a batch of nk items over NSTREAM streams: chunks of at most bmax items
 // (rounded to a multiple of 8 so the GEMM's XCD mapping applies), at least two chunks
@@ -310,13 +363,17 @@ static void lu_solve(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, co
 // NSTREAM streams (MSGPU_STREAMS overrides; the box exposes 4 hardware queues) and
 // chunks of at most MSGPU_CHUNK matrices.
 static constexpr int NSTREAM_MAX = 8;
-static int nstreams() {
+static int default_streams() {
     static int v = [] {
         const char* e = getenv("MSGPU_STREAMS");
         int x = e ? atoi(e) : 4;
         return std::max(1, std::min(x, NSTREAM_MAX));
     }();
     return v;
+}
+static int streams_for(int np, int prec) {
+    const int t = tune_for(np, prec).streams;
+    return t > 0 ? std::min(t, NSTREAM_MAX) : default_streams();
 }
 static int chunk_cap() {
     static int v = [] {
@@ -326,9 +383,9 @@ static int chunk_cap() {
     }();
     return v;
 }
-static int chunk_size(int nk, int bmax) {
-    const int ns = nstreams();
-    bmax = std::min(bmax, chunk_cap());
+static int chunk_size(int nk, int bmax, int np, int prec, int ns) {
+    const int tuned = tune_for(np, prec).chunk;
+    bmax = std::min(bmax, tuned > 0 ? tuned : chunk_cap());
     int nch = (nk + bmax - 1) / bmax;
     if (nch < ns && nk >= 32 * ns) nch = ns;
     else if (nch < 2 && nk >= 64) nch = 2;
@@ -337,11 +394,11 @@ static int chunk_size(int nk, int bmax) {
     return std::max(1, std::min(c, bmax));
 }
 
-struct Lanes {   // nstreams() leased streams; lane 0 owns the inputs, the others wait for them
+struct Lanes {   // ns leased streams; lane 0 owns the inputs, the others wait for them
     int ns;
     std::vector<Lease> l;
     hipEvent_t ev[NSTREAM_MAX];
-    Lanes() : ns(nstreams()) {
+    explicit Lanes(int n) : ns(std::max(1, std::min(n, NSTREAM_MAX))) {
         l.reserve(ns);
         for (int i = 0; i < ns; i++) l.emplace_back(true);
         for (int i = 0; i < ns; i++) HIPCHK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
@@ -374,6 +431,12 @@ struct Plan {
     int max_b = 0;
 };
 
+static int max_np(const std::vector<MediumView>& meds, int nk, const int* midx) {
+    int m = 0;
+    for (int i = 0; i < nk; i++) m = std::max(m, meds[midx ? midx[i] : 0].np);
+    return m;
+}
+
 static Plan make_plan(const std::vector<MediumView>& meds, int nk, const int* midx, size_t elem, int mats_per_item,
                       int ns) {
     Plan P;
@@ -386,7 +449,7 @@ static Plan make_plan(const std::vector<MediumView>& meds, int nk, const int* mi
         int g1 = g0;
         while (g1 < nk && np_of(P.order[g1]) == np) g1++;
         const int cnt = g1 - g0;
-        const int bsz = chunk_size(cnt, max_batch(np, elem, mats_per_item * ns));
+        const int bsz = chunk_size(cnt, max_batch(np, elem, mats_per_item * ns), np, elem == 16 ? 0 : 1, ns);
         for (int s0 = g0; s0 < g1; s0 += bsz) {
             const int B = std::min(bsz, g1 - s0);
             P.chunks.push_back({np, s0, B});
@@ -440,7 +503,7 @@ struct DeviceItems {
 template <class T>
 static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
                           cx<double>* out, double* stats) {
-    Lanes S;
+    Lanes S(streams_for(max_np(meds, nk, midx), prec_of<T>()));
     const Plan P = make_plan(meds, nk, midx, sizeof(cx<T>), 1, S.ns);
     DeviceItems D(meds, nk, midx, ks, P.order, S[0]);
     DBuf<cx<double>> dout(nk);
@@ -466,8 +529,8 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
         if (c > 0 && (int)c < nlane) HIPCHK(hipStreamWaitEvent(st, stag[c - 1], 0));
         timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st,
               [&] { launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st); });
-        factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, st,
-                  ((int)c + 1 < nlane) ? stag[c] : nullptr);
+        factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, tune_for(np, prec_of<T>()).rm == 1,
+                  st, ((int)c + 1 < nlane) ? stag[c] : nullptr);
         timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {
             launch_tracelog<T>(mats[ln].p, 0, D.items.p + ch.start, np, ms, ipiv[ln].p, dout.p + ch.start,
                                dstats.p + 2 * (size_t)ch.start, B, st);
@@ -520,18 +583,19 @@ int tracelog_points(const MediumView& mv, int nk, const cx<double>* ks, cx<doubl
 template <class T>
 static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
                     cx<double>* out) {
-    Lanes S;
+    Lanes S(streams_for(max_np(meds, nk, midx), prec_of<T>()));
     const Plan P = make_plan(meds, nk, midx, sizeof(cx<T>), 2, S.ns);
     DeviceItems D(meds, nk, midx, ks, P.order, S[0]);
     DBuf<cx<double>> dout(nk);
     S.fork();
     const int nlane = std::min<int>(S.ns, (int)P.chunks.size());
     std::vector<DBuf<cx<T>>> mats(nlane), rhs(nlane);
-    std::vector<DBuf<int>> ipiv(nlane), info(nlane), pmap(nlane);
+    std::vector<DBuf<int>> ipiv(nlane), info(nlane), pmap(nlane), perm(nlane);
     for (int i = 0; i < nlane; i++) {
         mats[i].alloc(P.mat_elems);
         rhs[i].alloc(P.mat_elems);
         ipiv[i].alloc(P.piv_ints);
+        perm[i].alloc(P.piv_ints);
         info[i].alloc(P.max_b);
         pmap[i].alloc(P.map_ints);
     }
@@ -543,9 +607,11 @@ static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx
         const MatItem* it = D.items.p + ch.start;
         launch_assemble_items<T>(it, np, 0, mats[ln].p, ms, B, st);
         launch_assemble_items<T>(it, np, 1, rhs[ln].p, ms, B, st);
-        factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, st);
-        lu_solve<T>(mats[ln].p, rhs[ln].p, np, np, B, pmap[ln].p, ms, st);
-        launch_diag_sum<T>(rhs[ln].p, 0, it, np, ms, dout.p + ch.start, B, st);
+        // tr(M^-1 M') = tr(P V) = sum_i V[i, perm[i]] with V from lu_solve_rm
+        factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, true, true, st);
+        lu_solve_rm<T>(mats[ln].p, rhs[ln].p, np, np, B, ms, st);
+        launch_build_perm(pmap[ln].p, np, NB, perm[ln].p, B, st);
+        launch_diag_sum<T>(rhs[ln].p, 0, it, np, ms, dout.p + ch.start, B, st, perm[ln].p);
     }
     S.join();
     std::vector<cx<double>> hout(nk);
@@ -567,6 +633,111 @@ int dk_log_det_multi(const std::vector<MediumView>& meds, int nk, const int* mid
 
 int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, int precision) {
     return dk_log_det_multi(std::vector<MediumView>{mv}, nk, nullptr, ks, out, precision);
+}
+
+// ------------------------------------------------------------------ autotuner
+// Times the batched trace-log of nitems matrices of padded size np on a synthetic
+// medium (d = 3, N = np scatterers uniform in the unit-density ball, hard spheres
+// alpha = 0.05, k on a grid over Re k in [5, 30] x Im k in [-2, 0]) for each
+// candidate and keeps the fastest, one parameter at a time: the swap + TRSM shape,
+// then the chunk size, then the stream count.  Every trial runs once untimed first.
+int autotune(int np, int prec, int nitems, double* report) {
+    if (np < 64 || np > 8192 || np % 64) { set_error("msgpu_autotune: N must give 64 <= Np <= 8192"); return -2; }
+    nitems = std::max(8, nitems);
+    // synthetic medium
+    std::vector<double> pos(3 * (size_t)np);
+    const double R = std::cbrt(np / (4.0 * kPi / 3.0));
+    uint64_t st = 88172645463325252ull;
+    auto uni = [&] { st ^= st << 13; st ^= st >> 7; st ^= st << 17; return (st >> 11) * (1.0 / 9007199254740992.0); };
+    for (int i = 0; i < np;) {
+        double x[3], r2 = 0;
+        for (int c = 0; c < 3; c++) { x[c] = R * (2 * uni() - 1); r2 += x[c] * x[c]; }
+        if (r2 > R * R) continue;
+        for (int c = 0; c < 3; c++) pos[3 * (size_t)i + c] = x[c];
+        i++;
+    }
+    MediumView mv;
+    mv.d = 3; mv.n = np; mv.np = np; mv.m = ScModel{0, 0.05, 0.0}; mv.pos = pos.data();
+    const int side = (int)std::ceil(std::sqrt((double)nitems));
+    std::vector<cx<double>> ks(nitems), out(nitems);
+    for (int l = 0; l < nitems; l++)
+        ks[l] = cx<double>{5.0 + 25.0 * (l % side) / std::max(1, side - 1), -2.0 * (l / side) / std::max(1, side - 1)};
+    const std::vector<MediumView> meds{mv};
+    auto trial = [&](Tune t) -> double {
+        {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            g_tune[{np, prec}] = t;
+        }
+        if (tracelog_multi(meds, nitems, nullptr, ks.data(), out.data(), nullptr, prec) < 0) return INFINITY;
+        auto t0 = std::chrono::steady_clock::now();
+        if (tracelog_multi(meds, nitems, nullptr, ks.data(), out.data(), nullptr, prec) < 0) return INFINITY;
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    Tune best;
+    best.chunk = chunk_cap();
+    best.streams = default_streams();
+    double best_ms = INFINITY;
+    for (int rm = 0; rm < 2; rm++)
+        for (int v = 0; v < swap_variants(); v++) {
+            Tune t = best;
+            t.swap = v;
+            t.rm = rm;
+            const double ms = trial(t);
+            if (ms < best_ms) { best_ms = ms; best = t; }
+        }
+    for (int c : {128, 256, 512}) {
+        Tune t = best;
+        t.chunk = c;
+        const double ms = trial(t);
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+    for (int ns : {1, 2, 4}) {
+        Tune t = best;
+        t.streams = ns;
+        const double ms = trial(t);
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune[{np, prec}] = best;
+    }
+    if (!std::isfinite(best_ms)) return -2;
+    if (report) {
+        report[0] = best.swap; report[1] = best.chunk; report[2] = best.streams; report[3] = best_ms;
+        report[4] = best.rm;
+    }
+    return 0;
+}
+
+int save_tuning(const char* path) {
+    FILE* f = fopen(path, "w");
+    if (!f) { set_error(std::string("msgpu_tune_save: cannot write ") + path); return -2; }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    fprintf(f, "# np precision swap_variant chunk streams row_major\n");
+    for (auto& kv : g_tune)
+        fprintf(f, "%d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.swap, kv.second.chunk,
+                kv.second.streams, kv.second.rm);
+    fclose(f);
+    return 0;
+}
+
+int load_tuning(const char* path) {
+    FILE* f = fopen(path, "r");
+    if (!f) { set_error(std::string("msgpu_tune_load: cannot read ") + path); return -2; }
+    char line[256];
+    int n = 0;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    while (fgets(line, sizeof line, f)) {
+        int a, b, c, d, e = 0, f6 = 0;
+        if (line[0] == '#') continue;
+        const int got = sscanf(line, "%d %d %d %d %d %d", &a, &b, &c, &d, &e, &f6);
+        if (got >= 4 && c >= 0 && c < swap_variants() && d >= 0 && e >= 0 && e <= NSTREAM_MAX && (f6 == 0 || f6 == 1)) {
+            g_tune[{a, b}] = Tune{c, d, e, f6};
+            n++;
+        }
+    }
+    fclose(f);
+    return n;
 }
 
 int upload_positions(const MediumView& mv, const double** dpos, size_t* token) {
@@ -599,23 +770,26 @@ int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>*
         DBuf<double> dpos((size_t)n * mv.d);
         DBuf<cx<double>> dk(1), ddiag(1), phi(np), x(np), r(np);
         DBuf<cx<double>> m64(ms);
-        DBuf<int> ipiv(np), info(1), pmap(pmap_ints(np, NB));
+        DBuf<int> ipiv(np), info(1), pmap(pmap_ints(np, NB)), perm(np);
         HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpyAsync(dk.p, &k, sizeof(k), hipMemcpyHostToDevice, L.st));
         launch_diag(dk.p, 1, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
         launch_incident(dpos.p, n, mv.d, np, k, source, 1.0, 0.0, phi.p, L.st);
         launch_assemble<double>(dpos.p, n, mv.d, np, dk.p, ddiag.p, nullptr, 0, m64.p, ms, 1, L.st);
         int steps = 0;
+        // row-major LU of the symmetric M (left interchanges applied); M^-1 v = P V
         if (!mixed) {
-            HIPCHK(hipMemcpyAsync(x.p, phi.p, sizeof(cx<double>) * np, hipMemcpyDeviceToDevice, L.st));
-            factor<double>(m64.p, np, 1, ipiv.p, info.p, pmap.p, false, L.st);
-            lu_solve<double>(m64.p, x.p, np, 1, 1, pmap.p, np, L.st);
+            factor<double>(m64.p, np, 1, ipiv.p, info.p, pmap.p, true, true, L.st);
+            launch_build_perm(pmap.p, np, NB, perm.p, 1, L.st);
+            lu_solve_rm<double>(m64.p, phi.p, np, 1, 1, np, L.st);
+            launch_scatter_rows<double>(x.p, phi.p, perm.p, np, 1, np, np, 1, L.st);
         } else {
             // complex64 LU, FP64 residuals: x <- x + M32^-1 (phi - M64 x)
             if (tol <= 0) tol = 1e-12;
-            DBuf<cx<float>> m32(ms), d32(np);
+            DBuf<cx<float>> m32(ms), d32(np), v32(np);
             launch_convert<double, float>(m64.p, m32.p, ms, L.st);
-            factor<float>(m32.p, np, 1, ipiv.p, info.p, pmap.p, false, L.st);
+            factor<float>(m32.p, np, 1, ipiv.p, info.p, pmap.p, true, true, L.st);
+            launch_build_perm(pmap.p, np, NB, perm.p, 1, L.st);
             HIPCHK(hipMemsetAsync(x.p, 0, sizeof(cx<double>) * np, L.st));
             std::vector<cx<double>> hphi(np), hr(np);
             HIPCHK(hipMemcpyAsync(hphi.data(), phi.p, sizeof(cx<double>) * np, hipMemcpyDeviceToHost, L.st));
@@ -626,8 +800,9 @@ int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>*
                 HIPCHK(hipMemcpyAsync(hr.data(), r.p, sizeof(cx<double>) * np, hipMemcpyDeviceToHost, L.st));
                 L.sync();
                 if (host_norm(hr, n) <= tol * nphi) break;
-                launch_convert<double, float>(r.p, d32.p, np, L.st);
-                lu_solve<float>(m32.p, d32.p, np, 1, 1, pmap.p, np, L.st);
+                launch_convert<double, float>(r.p, v32.p, np, L.st);
+                lu_solve_rm<float>(m32.p, v32.p, np, 1, 1, np, L.st);
+                launch_scatter_rows<float>(d32.p, v32.p, perm.p, np, 1, np, np, 1, L.st);
                 launch_axpy<float>(x.p, d32.p, np, L.st);
             }
         }
@@ -754,7 +929,7 @@ int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, i
                                     nrhs, hipMemcpyHostToDevice, L.st));
         }
         // factor without left interchanges, solve, then apply them for the LAPACK-format output
-        factor<double>(m.p, np, 1, dpiv.p, dinfo.p, pmap.p, false, L.st);
+        factor<double>(m.p, np, 1, dpiv.p, dinfo.p, pmap.p, false, false, L.st);
         if (tracelog) launch_tracelog<double>(m.p, n, nullptr, np, (long long)np * np, dpiv.p, tl.p, nullptr, 1, L.st);
         if (nrhs > 0) lu_solve<double>(m.p, x.p, np, nrhs, 1, pmap.p, np, L.st);
         for (int k0 = NB; k0 < np; k0 += NB) launch_swap_left<double>(m.p, np, (long long)np * np, pmap.p, k0, NB, 1, L.st);

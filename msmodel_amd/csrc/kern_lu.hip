@@ -45,11 +45,19 @@ template <> struct mfma<float> {
     static __device__ __forceinline__ int arow(int i) { return i; }
 };
 
+// Element (r, c) of an np x np matrix: column-major (RM = false) or row-major (RM =
+// true).  The row-major factorisation of a symmetric M is the column-pivoted one of
+// the same buffer read column-major: its row interchanges move contiguous rows.
+template <bool RM>
+__device__ __forceinline__ size_t at(int r, int c, int np) {
+    return RM ? (size_t)r * np + (size_t)c : (size_t)r + (size_t)c * np;
+}
+
 // ------------------------------------------------------------------ panel
 // pmap layout per matrix and panel q: [cnt, dst[2*nb], src[2*nb]] (1 + 4*nb ints).
 // One 1024-thread workgroup per matrix; row g of the current sub-panel (columns
 // [c0, c0+S)) lives in thread (g-c0) % PT, register row (g-c0) / PT.
-template <class T, int S, int R, int PT>
+template <class T, int S, int R, int PT, bool RM>
 __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int np, long long mstride,
                                                       int* __restrict__ ipiv_all, int* __restrict__ info_all,
                                                       int* __restrict__ pmap_all, int k0, int nb) {
@@ -83,7 +91,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             const int g = c0 + tid + PT * r;
 #pragma unroll
             for (int c = 0; c < S; c++)
-                x[r][c] = (g < np) ? M[(size_t)g + (size_t)(c0 + c) * np] : cx<T>{T(0), T(0)};
+                x[r][c] = (g < np) ? M[at<RM>(g, c0 + c, np)] : cx<T>{T(0), T(0)};
         }
         // Column loop with a rotating register file: at step jj, x[r][c] holds column
         // c0 + (jj + c) % S, so the active column is always x[r][0] and every register
@@ -178,7 +186,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             const int g = c0 + tid + PT * r;
             if (g < np)
 #pragma unroll
-                for (int c = 0; c < S; c++) M[(size_t)g + (size_t)(c0 + c) * np] = x[r][c];
+                for (int c = 0; c < S; c++) M[at<RM>(g, c0 + c, np)] = x[r][c];
         }
         if (tid < S) {
 #pragma unroll
@@ -208,9 +216,9 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             for (int u = 0; u < 4; u++) {
                 const int idx = tid + PT * u;
                 if (idx < pairs) {
-                    const int e = idx % cnt, ci = idx / cnt;
+                    const int e = RM ? idx / ncol : idx % cnt, ci = RM ? idx % ncol : idx / cnt;
                     const int col = ci < nleft ? k0 + ci : c0 + S + (ci - nleft);
-                    tmp[u] = M[(size_t)s_ss[e] + (size_t)col * np];
+                    tmp[u] = M[at<RM>(s_ss[e], col, np)];
                 }
             }
             __syncthreads();
@@ -218,9 +226,9 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             for (int u = 0; u < 4; u++) {
                 const int idx = tid + PT * u;
                 if (idx < pairs) {
-                    const int e = idx % cnt, ci = idx / cnt;
+                    const int e = RM ? idx / ncol : idx % cnt, ci = RM ? idx % ncol : idx / cnt;
                     const int col = ci < nleft ? k0 + ci : c0 + S + (ci - nleft);
-                    M[(size_t)s_sd[e] + (size_t)col * np] = tmp[u];
+                    M[at<RM>(s_sd[e], col, np)] = tmp[u];
                 }
             }
             __syncthreads();
@@ -229,13 +237,13 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             // U12' = L11'^-1 A12' for the panel columns right of the sub-panel
             if (tid < nright) {          // solved in LDS: the registers hold the sub-panel
                 const int col = c0 + S + tid;
-                for (int i = 0; i < S; i++) s_U[i][tid] = M[(size_t)(c0 + i) + (size_t)col * np];
+                for (int i = 0; i < S; i++) s_U[i][tid] = M[at<RM>(c0 + i, col, np)];
                 for (int i = 1; i < S; i++) {
                     cx<T> acc = s_U[i][tid];
                     for (int m = 0; m < i; m++) acc = acc - s_L[i][m] * s_U[m][tid];
                     s_U[i][tid] = acc;
                 }
-                for (int i = 0; i < S; i++) M[(size_t)(c0 + i) + (size_t)col * np] = s_U[i][tid];
+                for (int i = 0; i < S; i++) M[at<RM>(c0 + i, col, np)] = s_U[i][tid];
             }
             __syncthreads();
             // rank-S update of rows below the sub-panel, WR columns per round
@@ -248,14 +256,14 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
                     cx<T> av[WR];
 #pragma unroll
                     for (int w = 0; w < WR; w++)
-                        if (w0 + w < nright) av[w] = M[(size_t)g + (size_t)(c0 + S + w0 + w) * np];
+                        if (w0 + w < nright) av[w] = M[at<RM>(g, c0 + S + w0 + w, np)];
 #pragma unroll
                     for (int w = 0; w < WR; w++) {
                         if (w0 + w < nright) {
                             cx<T> a = av[w];
 #pragma unroll
                             for (int m = 0; m < S; m++) a = a - x[r][m] * s_U[m][w0 + w];
-                            M[(size_t)g + (size_t)(c0 + S + w0 + w) * np] = a;
+                            M[at<RM>(g, c0 + S + w0 + w, np)] = a;
                         }
                     }
                 }
@@ -278,13 +286,14 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
 // The register file is rotated by one after every 4 rows, so the row being
 // eliminated is always xv[0] (forward) or xv[15] (backward) and every register
 // index is a compile-time constant while the 16-step loop stays rolled.
-template <class T>
+template <class T, bool UNIT = true>
 __device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], const cx<T> (*sL)[65], int nb, int qq, int lane) {
 #pragma unroll 1
     for (int ib = 0; ib < 16; ib++) {
 #pragma unroll
         for (int iq = 0; iq < 4; iq++) {
             const int i = 4 * ib + iq;
+            if (!UNIT && qq == iq && i < nb) xv[0] = xv[0] / sL[i][i];
             const int src = (lane & ~3) | iq;
             const cx<T> xi{__shfl(xv[0].x, src, 64), __shfl(xv[0].y, src, 64)};
 #pragma unroll
@@ -300,14 +309,14 @@ __device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], const cx<T> (*
     }
 }
 
-template <class T>
+template <class T, bool UNIT = false>
 __device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], const cx<T> (*sL)[65], int qq, int lane) {
 #pragma unroll 1
     for (int ib = 15; ib >= 0; ib--) {
 #pragma unroll
         for (int iq = 3; iq >= 0; iq--) {
             const int i = 4 * ib + iq;
-            if (qq == iq) xv[15] = xv[15] / sL[i][i];
+            if (!UNIT && qq == iq) xv[15] = xv[15] / sL[i][i];
             const int src = (lane & ~3) | iq;
             const cx<T> xi{__shfl(xv[15].x, src, 64), __shfl(xv[15].y, src, 64)};
 #pragma unroll
@@ -324,14 +333,16 @@ __device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], const cx<T> (*sL)[6
 }
 
 // ------------------------------------------------------------------ swap + TRSM
-// grid.x = column tiles of SWT columns: the first nleft_t tiles cover [0, k0)
-// (LAPACK-format L), the rest cover [kend, np).  Right tiles also get
-// U12 = L11^-1 A12, 64 columns at a time against L11 staged once in LDS.
-constexpr int SWT = 256;
-template <class T>
+// grid.x = column tiles of TW columns: the first nleft_t tiles cover [0, k0)
+// (LAPACK-format L), the rest cover [kend, np).  The row map is applied RW columns
+// per round (every read of a round precedes its writes; up to 128 x RW scattered
+// elements in flight per workgroup).  Right tiles then get U12 = L11^-1 A12, 64
+// columns at a time against L11 staged once in LDS.
+template <class T, int TW, int RW, bool RM>
 __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, int np, long long mstride,
                                                         const int* __restrict__ pmap_all, int k0, int nb,
                                                         int nleft_t, int do_trsm) {
+    constexpr int U = (128 * RW + 255) / 256;
     __shared__ int s_dst[128], s_src[128];
     __shared__ cx<T> sL[64][65];
     const int b = blockIdx.y;
@@ -341,41 +352,42 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
     const int kend = min(k0 + nb, np);
     const int tid = threadIdx.x;
     const bool left = (int)blockIdx.x < nleft_t;
-    const int col0 = left ? blockIdx.x * SWT : kend + (blockIdx.x - nleft_t) * SWT;
-    const int colend = left ? min(col0 + SWT, k0) : min(col0 + SWT, np);
+    const int col0 = left ? blockIdx.x * TW : kend + (blockIdx.x - nleft_t) * TW;
+    const int colend = left ? min(col0 + TW, k0) : min(col0 + TW, np);
     const int ncol = colend - col0;
     const int cnt = pmap[0];
     for (int e = tid; e < cnt; e += 256) { s_dst[e] = pmap[1 + e]; s_src[e] = pmap[1 + 2 * nb + e]; }
     if (!left && do_trsm) {
         // unit-lower L11 (nb x nb) of the panel
         for (int e = tid; e < nb * nb; e += 256) {
-            const int i = e % nb, m = e / nb;
-            sL[i][m] = M[(size_t)(k0 + i) + (size_t)(k0 + m) * np];
+            const int i = RM ? e / nb : e % nb, m = RM ? e % nb : e / nb;
+            sL[i][m] = M[at<RM>(k0 + i, k0 + m, np)];
         }
     }
     __syncthreads();
-    // apply the row map, 8 columns per round (all reads of a column precede its writes)
-    for (int cb = 0; cb < ncol; cb += 8) {
-        const int pairs = cnt * min(8, ncol - cb);
-        cx<T> tmp[4];
+    if (cnt > 0) {
+        for (int cb = 0; cb < ncol; cb += RW) {
+            const int w = min(RW, ncol - cb), pairs = cnt * w;
+            cx<T> tmp[U];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int idx = tid + 256 * u;
-            if (idx < pairs) {
-                const int e = idx % cnt, c = col0 + cb + idx / cnt;
-                tmp[u] = M[(size_t)s_src[e] + (size_t)c * np];
+            for (int u = 0; u < U; u++) {
+                const int idx = tid + 256 * u;
+                if (idx < pairs) {
+                    const int e = RM ? idx / w : idx % cnt, c = col0 + cb + (RM ? idx % w : idx / cnt);
+                    tmp[u] = M[at<RM>(s_src[e], c, np)];
+                }
             }
-        }
-        __syncthreads();
+            __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int idx = tid + 256 * u;
-            if (idx < pairs) {
-                const int e = idx % cnt, c = col0 + cb + idx / cnt;
-                M[(size_t)s_dst[e] + (size_t)c * np] = tmp[u];
+            for (int u = 0; u < U; u++) {
+                const int idx = tid + 256 * u;
+                if (idx < pairs) {
+                    const int e = RM ? idx / w : idx % cnt, c = col0 + cb + (RM ? idx % w : idx / cnt);
+                    M[at<RM>(s_dst[e], c, np)] = tmp[u];
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (left || !do_trsm) return;
     // TRSM: 4 lanes per column; lane q of a group holds rows i = 4m + q, m < nb/4
@@ -387,18 +399,45 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
 #pragma unroll
         for (int m = 0; m < 16; m++) {
             const int i = 4 * m + qq;
-            xv[m] = (live && i < nb) ? M[(size_t)(k0 + i) + (size_t)col * np] : cx<T>{T(0), T(0)};
+            xv[m] = (live && i < nb) ? M[at<RM>(k0 + i, col, np)] : cx<T>{T(0), T(0)};
         }
         lower_unit_solve<T>(xv, sL, nb, qq, lane);
         if (live) {
 #pragma unroll
             for (int m = 0; m < 16; m++) {
                 const int i = 4 * m + qq;
-                if (i < nb) M[(size_t)(k0 + i) + (size_t)col * np] = xv[m];
+                if (i < nb) M[at<RM>(k0 + i, col, np)] = xv[m];
             }
         }
     }
 }
+
+// Tile width and round width of the swap + TRSM variants (the autotuner picks one).
+struct SwapShape { int tw, rw; };
+constexpr SwapShape kSwapShapes[] = {{256, 8}, {128, 16}, {64, 16}, {64, 32}, {128, 32}, {256, 32}};
+constexpr int kNumSwapShapes = sizeof(kSwapShapes) / sizeof(kSwapShapes[0]);
+
+template <class T, bool RM>
+static void swap_go_rm(int variant, dim3 grid, hipStream_t st, cx<T>* A, int np, long long mstride, const int* pmap,
+                       int k0, int nb, int nleft_t, int do_trsm) {
+    switch (variant) {
+        case 1: hipLaunchKernelGGL((swap_trsm_kernel<T, 128, 16, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
+        case 2: hipLaunchKernelGGL((swap_trsm_kernel<T, 64, 16, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
+        case 3: hipLaunchKernelGGL((swap_trsm_kernel<T, 64, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
+        case 4: hipLaunchKernelGGL((swap_trsm_kernel<T, 128, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
+        case 5: hipLaunchKernelGGL((swap_trsm_kernel<T, 256, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
+        default: hipLaunchKernelGGL((swap_trsm_kernel<T, 256, 8, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
+    }
+}
+
+template <class T>
+static void swap_go(int variant, dim3 grid, hipStream_t st, cx<T>* A, int np, long long mstride, const int* pmap,
+                    int k0, int nb, int nleft_t, int do_trsm, bool rm) {
+    if (rm) swap_go_rm<T, true>(variant, grid, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm);
+    else swap_go_rm<T, false>(variant, grid, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm);
+}
+
+int swap_variants() { return kNumSwapShapes; }
 
 // ------------------------------------------------------------------ GEMM: C -= A B
 constexpr int GT = 64;     // complex tile edge (rows and columns)
@@ -589,8 +628,10 @@ __global__ void gather_rows_kernel(cx<T>* __restrict__ dst, const cx<T>* __restr
 
 // ------------------------------------------------------------------ triangular block solve
 // Rows [j0, j0+64) of B (np x ncol, ld np) against the 64x64 diagonal block of the
-// factor: UPPER=0 unit-lower forward, UPPER=1 non-unit upper backward.
-template <class T, int UPPER>
+// factor read column-major: UPPER=0 lower forward, UPPER=1 upper backward, with a
+// unit or stored diagonal (UNIT).  Column-major LU: unit lower, non-unit upper; the
+// row-major LU of a symmetric matrix read column-major: non-unit lower, unit upper.
+template <class T, int UPPER, bool UNIT>
 __global__ __launch_bounds__(256) void trsm_block_kernel(const cx<T>* __restrict__ LU, cx<T>* __restrict__ B,
                                                          int np, int ncol, int j0, long long slu, long long sb) {
     __shared__ cx<T> sL[64][65];
@@ -611,25 +652,26 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(const cx<T>* __restrict
         const int i = 4 * m + qq;
         xv[m] = live ? X[(size_t)(j0 + i) + (size_t)col * np] : cx<T>{T(0), T(0)};
     }
-    if (!UPPER) lower_unit_solve<T>(xv, sL, 64, qq, lane);
-    else upper_solve<T>(xv, sL, qq, lane);
+    if (!UPPER) lower_unit_solve<T, UNIT>(xv, sL, 64, qq, lane);
+    else upper_solve<T, UNIT>(xv, sL, qq, lane);
     if (live) {
 #pragma unroll
         for (int m = 0; m < 16; m++) X[(size_t)(j0 + 4 * m + qq) + (size_t)col * np] = xv[m];
     }
 }
 
-// out[b] = sum_{i<n} X[b][i,i] in FP64.
+// out[b] = sum_{i<n} X[b][i, p(i)] in FP64, p(i) = perm[b][i] (perm given, ld = np) or i.
 template <class T>
 __global__ __launch_bounds__(256) void diag_sum_kernel(const cx<T>* __restrict__ X, int n,
                                                        const MatItem* __restrict__ items, int ld, long long sx,
-                                                       cx<double>* __restrict__ out) {
+                                                       cx<double>* __restrict__ out, const int* __restrict__ perm) {
     __shared__ double sre[256], sim[256];
     const int b = blockIdx.x, tid = threadIdx.x;
     if (items) n = items[b].n;
     double re = 0.0, im = 0.0;
     for (int i = tid; i < n; i += 256) {
-        const cx<T> v = X[(size_t)b * sx + (size_t)i + (size_t)i * ld];
+        const int c = perm ? perm[(size_t)b * ld + i] : i;
+        const cx<T> v = X[(size_t)b * sx + (size_t)i + (size_t)c * ld];
         re += (double)v.x;
         im += (double)v.y;
     }
@@ -706,37 +748,42 @@ __global__ void pad_identity_kernel(cx<T>* __restrict__ A, int n, int np, long l
 // per precision and padded size from measurements of the panel microbenchmark.
 template <class T, int S, int R, int PT>
 static void panel_go(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb,
-                     int nbatch, hipStream_t st) {
-    hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT>), dim3(nbatch), dim3(PT), sizeof(int) * (size_t)np, st, A, np,
-                       mstride, ipiv, info, pmap, k0, nb);
+                     int nbatch, hipStream_t st, bool rm) {
+    if (rm)
+        hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, true>), dim3(nbatch), dim3(PT), sizeof(int) * (size_t)np, st,
+                           A, np, mstride, ipiv, info, pmap, k0, nb);
+    else
+        hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, false>), dim3(nbatch), dim3(PT), sizeof(int) * (size_t)np,
+                           st, A, np, mstride, ipiv, info, pmap, k0, nb);
 }
 
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
-                  hipStream_t st) {
+                  hipStream_t st, bool rm) {
     constexpr bool D = sizeof(T) == 8;
     if (np <= 1024) {
-        if (D) panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
-        else panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
+        if (D) panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
     } else if (np <= 2048) {
-        if (D) panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
-        else panel_go<T, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
+        if (D) panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else panel_go<T, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
     } else if (np <= 4096) {
-        panel_go<T, 4, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
+        panel_go<T, 4, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
     } else {
-        panel_go<T, 2, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st);
+        panel_go<T, 2, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
     }
 }
 
 template <class T>
 void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,
-                      int nbatch, hipStream_t st) {
+                      int nbatch, hipStream_t st, int variant, bool rm) {
+    if (variant < 0 || variant >= kNumSwapShapes) variant = 0;
+    const int tw = kSwapShapes[variant].tw;
     const int kend = std::min(k0 + nb, np);
-    const int nleft_t = apply_left ? (k0 + SWT - 1) / SWT : 0;
-    const int nright_t = (np - kend + SWT - 1) / SWT;
+    const int nleft_t = apply_left ? (k0 + tw - 1) / tw : 0;
+    const int nright_t = (np - kend + tw - 1) / tw;
     if (nleft_t + nright_t == 0) return;
-    hipLaunchKernelGGL(swap_trsm_kernel<T>, dim3(nleft_t + nright_t, nbatch), dim3(256), 0, st, A, np, mstride, pmap,
-                       k0, nb, nleft_t, 1);
+    swap_go<T>(variant, dim3(nleft_t + nright_t, nbatch), st, A, np, mstride, pmap, k0, nb, nleft_t, 1, rm);
 }
 
 template <class T>
@@ -769,18 +816,43 @@ void launch_gather_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, i
 
 template <class T>
 void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int upper, long long slu, long long sb,
-                       int nbatch, hipStream_t st) {
+                       int nbatch, hipStream_t st, bool rm) {
     dim3 grid((ncol + 63) / 64, nbatch);
-    if (upper)
-        hipLaunchKernelGGL((trsm_block_kernel<T, 1>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
+    if (upper && !rm)
+        hipLaunchKernelGGL((trsm_block_kernel<T, 1, false>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
+    else if (upper)
+        hipLaunchKernelGGL((trsm_block_kernel<T, 1, true>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
+    else if (!rm)
+        hipLaunchKernelGGL((trsm_block_kernel<T, 0, true>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
     else
-        hipLaunchKernelGGL((trsm_block_kernel<T, 0>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
+        hipLaunchKernelGGL((trsm_block_kernel<T, 0, false>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
+}
+
+// dst[b][perm[i], c] = src[b][i, c] for i < np, c < ncol (column-major, ld np).
+template <class T>
+__global__ void scatter_rows_kernel(cx<T>* __restrict__ dst, const cx<T>* __restrict__ src,
+                                    const int* __restrict__ perm_all, int np, int ncol, long long sdst,
+                                    long long ssrc) {
+    const int b = blockIdx.z;
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)np * ncol) return;
+    const int i = (int)(e % np), c = (int)(e / np);
+    const int* perm = perm_all + (size_t)b * np;
+    dst[(size_t)b * sdst + (size_t)perm[i] + (size_t)c * np] = src[(size_t)b * ssrc + (size_t)i + (size_t)c * np];
+}
+
+template <class T>
+void launch_scatter_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, int ncol, long long sdst,
+                         long long ssrc, int nbatch, hipStream_t st) {
+    const long long tot = (long long)np * ncol;
+    hipLaunchKernelGGL(scatter_rows_kernel<T>, dim3((unsigned)((tot + 255) / 256), 1, nbatch), dim3(256), 0, st, dst,
+                       src, perm, np, ncol, sdst, ssrc);
 }
 
 template <class T>
 void launch_diag_sum(const cx<T>* X, int n, const MatItem* items, int ld, long long sx, cx<double>* out, int nbatch,
-                     hipStream_t st) {
-    hipLaunchKernelGGL(diag_sum_kernel<T>, dim3(nbatch), dim3(256), 0, st, X, n, items, ld, sx, out);
+                     hipStream_t st, const int* perm) {
+    hipLaunchKernelGGL(diag_sum_kernel<T>, dim3(nbatch), dim3(256), 0, st, X, n, items, ld, sx, out, perm);
 }
 
 template <class T1, class T2>
@@ -809,10 +881,9 @@ void launch_apply_map(cx<T>* X, int np, int ncol, const int* pmap, int nb, int q
 template <class T>
 void launch_swap_left(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int nbatch,
                       hipStream_t st) {
-    const int nleft_t = (k0 + SWT - 1) / SWT;
+    const int nleft_t = (k0 + 255) / 256;
     if (nleft_t == 0) return;
-    hipLaunchKernelGGL(swap_trsm_kernel<T>, dim3(nleft_t, nbatch), dim3(256), 0, st, A, np, mstride, pmap, k0, nb,
-                       nleft_t, 0);
+    swap_go<T>(0, dim3(nleft_t, nbatch), st, A, np, mstride, pmap, k0, nb, nleft_t, 0, false);
 }
 
 template <class T>
@@ -823,8 +894,8 @@ void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st) {
 }
 
 #define MSG_INST(T)                                                                                                   \
-    template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t);              \
-    template void launch_swap_trsm<T>(cx<T>*, int, long long, const int*, int, int, int, int, hipStream_t);           \
+    template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t, bool);        \
+    template void launch_swap_trsm<T>(cx<T>*, int, long long, const int*, int, int, int, int, hipStream_t, int, bool);\
     template void launch_gemm_sub<T>(cx<T>*, const cx<T>*, const cx<T>*, int, int, int, int, int, int, long long,      \
                                      long long, long long, int, hipStream_t);                                          \
     template void launch_tracelog<T>(const cx<T>*, int, const MatItem*, int, long long, const int*, cx<double>*,     \
@@ -832,9 +903,11 @@ void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st) {
     template void launch_gather_rows<T>(cx<T>*, const cx<T>*, const int*, int, int, long long, long long, int,       \
                                         hipStream_t);                                                                  \
     template void launch_trsm_block<T>(const cx<T>*, cx<T>*, int, int, int, int, long long, long long, int,           \
-                                       hipStream_t);                                                                   \
+                                       hipStream_t, bool);                                                             \
+    template void launch_scatter_rows<T>(cx<T>*, const cx<T>*, const int*, int, int, long long, long long, int,      \
+                                         hipStream_t);                                                                 \
     template void launch_diag_sum<T>(const cx<T>*, int, const MatItem*, int, long long, cx<double>*, int,             \
-                                     hipStream_t);                                                                     \
+                                     hipStream_t, const int*);                                                         \
     template void launch_axpy<T>(cx<double>*, const cx<T>*, int, hipStream_t);                                       \
     template void launch_apply_map<T>(cx<T>*, int, int, const int*, int, int, long long, int, hipStream_t);           \
     template void launch_swap_left<T>(cx<T>*, int, long long, const int*, int, int, int, hipStream_t);                \

@@ -36,12 +36,15 @@ void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double
                  double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi, hipStream_t st);
 
 // kern_lu.hip
+// rm: the matrices are stored row-major (see at<> in kern_lu.hip)
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
-                  hipStream_t st);
+                  hipStream_t st, bool rm = false);
+// variant: tile shape index < swap_variants() (0 = 256-column tiles, 8-column rounds)
 template <class T>
 void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,
-                      int nbatch, hipStream_t st);
+                      int nbatch, hipStream_t st, int variant = 0, bool rm = false);
+int swap_variants();
 template <class T>
 void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int k, int lda, int ldb, int ldc,
                      long long sC, long long sA, long long sB, int nbatch, hipStream_t st);
@@ -53,12 +56,19 @@ void launch_build_perm(const int* pmap, int np, int nb, int* perm, int nbatch, h
 template <class T>
 void launch_gather_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, int ncol, long long sdst,
                         long long ssrc, int nbatch, hipStream_t st);
+// rm = false: unit lower / non-unit upper (column-major LU); rm = true: non-unit lower /
+// unit upper (row-major LU of a symmetric matrix read column-major)
 template <class T>
 void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int upper, long long slu, long long sb,
-                       int nbatch, hipStream_t st);
+                       int nbatch, hipStream_t st, bool rm = false);
+// dst[perm[i], c] = src[i, c]
+template <class T>
+void launch_scatter_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, int ncol, long long sdst,
+                         long long ssrc, int nbatch, hipStream_t st);
+// out[b] = sum_{i<n} X[i, perm[i]] (perm null: X[i, i])
 template <class T>
 void launch_diag_sum(const cx<T>* X, int n, const MatItem* items, int ld, long long sx, cx<double>* out, int nbatch,
-                     hipStream_t st);
+                     hipStream_t st, const int* perm = nullptr);
 template <class T1, class T2>
 void launch_convert(const cx<T1>* x, cx<T2>* y, long long count, hipStream_t st);
 void launch_residual(const cx<double>* M, const cx<double>* a, const cx<double>* phi, int np, cx<double>* r,

@@ -1,0 +1,62 @@
+"""The autotuner and the tuning table: every tunable (swap + TRSM tile shape, chunk
+size, stream count) only moves data or changes the launch shape, so results are
+bit-identical whatever the table holds."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def G():
+    import msmodel_amd as G
+    G.init()
+    return G
+
+
+def test_autotune_report_and_results(G):
+    med = G.Medium(3, 300, G.HARDSPHERE, (0.05,)).fill(3)
+    ks = np.linspace(5, 30, 40) - 1j * np.linspace(0, 2, 40)
+    before, _ = G.tracelog_batch(med, ks, G.FP64)
+    dk_before = G.dk_log_det_batch(med, ks[:6], G.FP64)
+    rep = G.autotune(300, G.FP64, 64)
+    assert 0 <= rep["swap_variant"] < 6 and rep["chunk"] in (128, 256, 512) and rep["streams"] in (1, 2, 4)
+    assert rep["row_major"] in (0, 1)
+    assert rep["best_ms"] > 0
+    after, _ = G.tracelog_batch(med, ks, G.FP64)
+    if rep["row_major"] == 0:
+        assert np.array_equal(before, after)
+    else:
+        assert np.abs(after.real - before.real).max() <= 1e-10 * max(1, np.abs(before.real).max())
+    assert np.array_equal(dk_before, G.dk_log_det_batch(med, ks[:6], G.FP64))
+
+
+@pytest.mark.parametrize("prec_mode", ["FP64", "FP32"])
+def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
+    mode = getattr(G, prec_mode)
+    prec = 0 if mode == G.FP64 else 1
+    med = G.Medium(3, 330, G.HARDSPHERE, (0.05,)).fill(5)    # Np = 384: six panels
+    ks = np.linspace(6, 25, 24) - 0.5j
+    ref = None
+    for v in range(6):
+        for chunk, streams, rm in ((256, 4, 0), (16, 1, 0), (256, 2, 1), (16, 1, 1)):
+            path = os.path.join(str(tmp_path), "tune_%d_%d_%d.txt" % (v, chunk, rm))
+            with open(path, "w") as f:
+                f.write("384 %d %d %d %d %d\n" % (prec, v, chunk, streams, rm))
+            assert G.tune_load(path) == 1
+            out, _ = G.tracelog_batch(med, ks, mode)
+            if ref is None:
+                ref = out
+            if rm == 0:
+                assert np.array_equal(out, ref), (v, chunk, streams)
+            else:   # the row-major factorisation rounds differently: parity tolerance
+                tol = 1e-10 if prec == 0 else 1e-4
+                assert np.abs(out.real - ref.real).max() <= tol * max(1, np.abs(ref.real).max())
+                dim = (330 * (out.imag - ref.imag) + np.pi) % (2 * np.pi) - np.pi
+                assert np.abs(dim).max() <= (1e-8 if prec == 0 else 1e-2)
+    path = os.path.join(str(tmp_path), "tune_restore.txt")
+    with open(path, "w") as f:
+        f.write("384 %d 0 256 4\n" % prec)
+    G.tune_load(path)
