@@ -254,11 +254,12 @@ static int max_batch(int np, size_t elem, int mats_per_item) {
 
 // ------------------------------------------------------------------ tuning table
 // Per (padded size, precision): the swap + TRSM tile shape, the chunk size, the
-// stream count and the storage order of the trace-log factorisation (rm: 1 row-major,
-// 0 column-major).  Filled by autotune() or load_tuning(); an entry of 0 keeps the
-// built-in choice (variant 0, MSGPU_CHUNK, MSGPU_STREAMS, column-major).
+// stream count, the storage order of the trace-log factorisation (rm: 1 row-major,
+// 0 column-major) and the panel shape.  Filled by autotune() or load_tuning(); an
+// entry of 0 keeps the built-in choice (variant 0, MSGPU_CHUNK, MSGPU_STREAMS,
+// column-major, default panel).
 struct Tune {
-    int swap = 0, chunk = 0, streams = 0, rm = 0;
+    int swap = 0, chunk = 0, streams = 0, rm = 0, panel = 0;
 };
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, int>, Tune> g_tune;
@@ -289,11 +290,12 @@ static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* p
     if (np > 8192) throw std::runtime_error("matrices above N = 8192 are not supported by the device LU");
     HIPCHK(hipMemsetAsync(info, 0, sizeof(int) * nbatch, st));
     const double es = sizeof(cx<T>);
-    const int swapv = tune_for(np, prec_of<T>()).swap;
+    const Tune tn = tune_for(np, prec_of<T>());
+    const int swapv = tn.swap, panelv = tn.panel;
     for (int k0 = 0; k0 < np; k0 += NB) {
         const int kend = k0 + NB, rest = np - kend;
         timed(0, 8.0 * nbatch * ((double)(np - k0) * NB * NB / 2), es * nbatch * 2.0 * (np - k0) * NB, st,
-              [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st, rm); });
+              [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st, rm, panelv); });
         if (k0 == 0 && after_first_panel) HIPCHK(hipEventRecord(after_first_panel, st));
         timed(1, 8.0 * nbatch * (double)NB * NB / 2 * rest, es * nbatch * 2.0 * NB * rest, st,
               [&] { launch_swap_trsm<T>(mats, np, ms, pmap, k0, NB, left ? 1 : 0, nbatch, st, swapv, rm); });
@@ -685,6 +687,12 @@ int autotune(int np, int prec, int nitems, double* report) {
             const double ms = trial(t);
             if (ms < best_ms) { best_ms = ms; best = t; }
         }
+    for (int pv = 1; pv < kPanelVariants; pv++) {
+        Tune t = best;
+        t.panel = pv;
+        const double ms = trial(t);
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
     for (int c : {128, 256, 512}) {
         Tune t = best;
         t.chunk = c;
@@ -705,6 +713,7 @@ int autotune(int np, int prec, int nitems, double* report) {
     if (report) {
         report[0] = best.swap; report[1] = best.chunk; report[2] = best.streams; report[3] = best_ms;
         report[4] = best.rm;
+        report[5] = best.panel;
     }
     return 0;
 }
@@ -713,10 +722,10 @@ int save_tuning(const char* path) {
     FILE* f = fopen(path, "w");
     if (!f) { set_error(std::string("msgpu_tune_save: cannot write ") + path); return -2; }
     std::lock_guard<std::mutex> g(g_tune_mu);
-    fprintf(f, "# np precision swap_variant chunk streams row_major\n");
+    fprintf(f, "# np precision swap_variant chunk streams row_major panel_variant\n");
     for (auto& kv : g_tune)
-        fprintf(f, "%d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.swap, kv.second.chunk,
-                kv.second.streams, kv.second.rm);
+        fprintf(f, "%d %d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.swap, kv.second.chunk,
+                kv.second.streams, kv.second.rm, kv.second.panel);
     fclose(f);
     return 0;
 }
@@ -728,11 +737,12 @@ int load_tuning(const char* path) {
     int n = 0;
     std::lock_guard<std::mutex> g(g_tune_mu);
     while (fgets(line, sizeof line, f)) {
-        int a, b, c, d, e = 0, f6 = 0;
+        int a, b, c, d, e = 0, f6 = 0, g7 = 0;
         if (line[0] == '#') continue;
-        const int got = sscanf(line, "%d %d %d %d %d %d", &a, &b, &c, &d, &e, &f6);
-        if (got >= 4 && c >= 0 && c < swap_variants() && d >= 0 && e >= 0 && e <= NSTREAM_MAX && (f6 == 0 || f6 == 1)) {
-            g_tune[{a, b}] = Tune{c, d, e, f6};
+        const int got = sscanf(line, "%d %d %d %d %d %d %d", &a, &b, &c, &d, &e, &f6, &g7);
+        if (got >= 4 && c >= 0 && c < swap_variants() && d >= 0 && e >= 0 && e <= NSTREAM_MAX && (f6 == 0 || f6 == 1) &&
+            g7 >= 0 && g7 < kPanelVariants) {
+            g_tune[{a, b}] = Tune{c, d, e, f6, g7};
             n++;
         }
     }

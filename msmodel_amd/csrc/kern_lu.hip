@@ -215,6 +215,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int idx = tid + PT * u;
+                tmp[u] = cx<T>{T(0), T(0)};     // written on every path: the array stays in registers
                 if (idx < pairs) {
                     const int e = RM ? idx / ncol : idx % cnt, ci = RM ? idx % ncol : idx / cnt;
                     const int col = ci < nleft ? k0 + ci : c0 + S + (ci - nleft);
@@ -372,6 +373,7 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int idx = tid + 256 * u;
+                tmp[u] = cx<T>{T(0), T(0)};     // written on every path: the array stays in registers
                 if (idx < pairs) {
                     const int e = RM ? idx / w : idx % cnt, c = col0 + cb + (RM ? idx % w : idx / cnt);
                     tmp[u] = M[at<RM>(s_src[e], c, np)];
@@ -757,20 +759,36 @@ static void panel_go(cx<T>* A, int np, long long mstride, int* ipiv, int* info, 
                            st, A, np, mstride, ipiv, info, pmap, k0, nb);
 }
 
+// Panel shapes <S, R, PT> per size range, tried by the autotuner; variant 0 is the
+// default.  PT x R must cover np rows.  The 1024-thread shapes are limited to 128
+// VGPRs per lane; the 512-thread ones may use 256.
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
-                  hipStream_t st, bool rm) {
+                  hipStream_t st, bool rm, int variant) {
     constexpr bool D = sizeof(T) == 8;
+    if (variant < 0 || variant >= kPanelVariants) variant = 0;
     if (np <= 1024) {
-        if (D) panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (variant == 1) {
+            if (D) panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+            else panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        } else if (variant == 2) {
+            panel_go<T, 4, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        } else {
+            if (D) panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+            else panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        }
     } else if (np <= 2048) {
-        if (D) panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (variant == 1) panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else if (variant == 2) panel_go<T, 4, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
         else panel_go<T, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-    } else if (np <= 4096) {
-        panel_go<T, 4, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+    } else if (np <= 4096) {     // complex128 default: the 512-thread shape, which does not spill
+        if (variant == 1) panel_go<T, 2, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else if ((variant == 0) == D) panel_go<T, 4, 8, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else panel_go<T, 4, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
     } else {
-        panel_go<T, 2, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (variant == 1) panel_go<T, 1, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else if ((variant == 0) == D) panel_go<T, 2, 16, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else panel_go<T, 2, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
     }
 }
 
@@ -894,7 +912,7 @@ void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st) {
 }
 
 #define MSG_INST(T)                                                                                                   \
-    template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t, bool);        \
+    template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t, bool, int);   \
     template void launch_swap_trsm<T>(cx<T>*, int, long long, const int*, int, int, int, int, hipStream_t, int, bool);\
     template void launch_gemm_sub<T>(cx<T>*, const cx<T>*, const cx<T>*, int, int, int, int, int, int, long long,      \
                                      long long, long long, int, hipStream_t);                                          \

@@ -36,10 +36,12 @@ void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double
                  double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi, hipStream_t st);
 
 // kern_lu.hip
-// rm: the matrices are stored row-major (see at<> in kern_lu.hip)
+// rm: the matrices are stored row-major (see at<> in kern_lu.hip); variant: panel
+// shape index < kPanelVariants (0 = default for the size range)
+constexpr int kPanelVariants = 3;
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
-                  hipStream_t st, bool rm = false);
+                  hipStream_t st, bool rm = false, int variant = 0);
 // variant: tile shape index < swap_variants() (0 = 256-column tiles, 8-column rounds)
 template <class T>
 void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,

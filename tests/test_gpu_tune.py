@@ -23,7 +23,7 @@ def test_autotune_report_and_results(G):
     dk_before = G.dk_log_det_batch(med, ks[:6], G.FP64)
     rep = G.autotune(300, G.FP64, 64)
     assert 0 <= rep["swap_variant"] < 6 and rep["chunk"] in (128, 256, 512) and rep["streams"] in (1, 2, 4)
-    assert rep["row_major"] in (0, 1)
+    assert rep["row_major"] in (0, 1) and rep["panel_variant"] in (0, 1, 2)
     assert rep["best_ms"] > 0
     after, _ = G.tracelog_batch(med, ks, G.FP64)
     if rep["row_major"] == 0:
@@ -41,10 +41,10 @@ def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
     ks = np.linspace(6, 25, 24) - 0.5j
     ref = None
     for v in range(6):
-        for chunk, streams, rm in ((256, 4, 0), (16, 1, 0), (256, 2, 1), (16, 1, 1)):
+        for chunk, streams, rm, pv in ((256, 4, 0, 0), (16, 1, 0, 1), (256, 2, 1, 2), (16, 1, 1, v % 3)):
             path = os.path.join(str(tmp_path), "tune_%d_%d_%d.txt" % (v, chunk, rm))
             with open(path, "w") as f:
-                f.write("384 %d %d %d %d %d\n" % (prec, v, chunk, streams, rm))
+                f.write("384 %d %d %d %d %d %d\n" % (prec, v, chunk, streams, rm, pv))
             assert G.tune_load(path) == 1
             out, _ = G.tracelog_batch(med, ks, mode)
             if ref is None:
