@@ -181,6 +181,41 @@ def uneven_search(G, nmed, per, nmin, nmax):
             "sample": "%d media x %d starts (the full search runs 10000 starts)" % (nmed, per)}
 
 
+def small_map(G, steps):
+    """The reference's CPU-runnable map: d=3, N=200 in a ball of radius 1 (seed 1),
+    alpha=0.1, 128x64 k over Re[1,20] x Im[-1,0], complex128; with the chunk pipeline
+    launched eagerly and replayed from captured HIP graphs (tuned launch shapes)."""
+    import tempfile
+    med = G.Medium(3, 200, G.HARDSPHERE, (0.1,)).fill(1, radius=1.0)
+    ks = kgrid(128, 64, 1.0, 20.0, -1.0, 0.0)
+    tuned = G.autotune(200, G.FP64, 1024)
+    out = {"tuned": tuned}
+    with tempfile.TemporaryDirectory() as tmp:
+        for graph in (0, 1):
+            path = os.path.join(tmp, "t.txt")
+            with open(path, "w") as f:
+                f.write("256 0 %d %d %d %d %d %d\n" % (tuned["swap_variant"], tuned["chunk"], tuned["streams"],
+                                                        tuned["row_major"], tuned["panel_variant"], graph))
+            G.tune_load(path)
+            G.tracelog_batch(med, ks, G.FP64)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                res, _ = G.tracelog_batch(med, ks, G.FP64)
+            out["graph" if graph else "eager"] = steps * ks.size / (time.perf_counter() - t0)
+            out["tracelog_%d" % graph] = res
+    same = bool(np.array_equal(out.pop("tracelog_0"), out.pop("tracelog_1")))
+    # restore the tuned choice
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "t.txt")
+        with open(path, "w") as f:
+            f.write("256 0 %d %d %d %d %d %d\n" % (tuned["swap_variant"], tuned["chunk"], tuned["streams"],
+                                                    tuned["row_major"], tuned["panel_variant"], tuned["graph"]))
+        G.tune_load(path)
+    return {"workload": "tracelog map d=3 N=200 ball R=1 seed 1 hardsphere 0.1, 128x64 k over Re[1,20] x Im[-1,0]",
+            "evals_per_s_eager": out["eager"], "evals_per_s_graph": out["graph"], "graph_equals_eager": same,
+            "autotune": tuned}
+
+
 def wavefunction_case(G):
     """Solve + psi grid: d=3, N=1000 in a ball of radius 5 (seed 2), alpha=0.1, plane
     wave at real k=10, psi on 1024x1024 in the (x0, x1) plane over [-6, 6]^2."""
@@ -217,6 +252,7 @@ def main():
     ap.add_argument("--uneven-starts", type=int, default=4, help="starts per medium of the uneven search")
     ap.add_argument("--no-wave", action="store_true", help="skip the solve + 1024^2 psi case")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shapes (no autotuning)")
+    ap.add_argument("--no-small", action="store_true", help="skip the N=200 map (eager vs HIP graph)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run this many device copies of 1 GiB (calibrates the HBM byte counters)")
@@ -274,6 +310,7 @@ def main():
     serving = serving_stream(G, args.requests, args.rate, 100, args.serve_nmax) if args.requests > 0 else None
     uneven = uneven_search(G, args.uneven, args.uneven_starts, 200, 2000) if args.uneven > 0 else None
     wave = None if args.no_wave else wavefunction_case(G)
+    small = None if args.no_small else small_map(G, 3)
 
     mixed_ok = float(dtl.max()) <= 1e-8
     if mixed_ok and mixed_rate > fp64_batched:
@@ -313,6 +350,7 @@ def main():
         "serving": serving,
         "uneven_resonance_search": uneven,
         "solve_wavefunction_n1000": wave,
+        "map_n200": small,
         "kernel_time_fraction": breakdown,
         "autotune": tuned,
         "roofline": roofline,

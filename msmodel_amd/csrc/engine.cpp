@@ -9,9 +9,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "capi_internal.h"
@@ -255,11 +257,12 @@ static int max_batch(int np, size_t elem, int mats_per_item) {
 // ------------------------------------------------------------------ tuning table
 // Per (padded size, precision): the swap + TRSM tile shape, the chunk size, the
 // stream count, the storage order of the trace-log factorisation (rm: 1 row-major,
-// 0 column-major) and the panel shape.  Filled by autotune() or load_tuning(); an
-// entry of 0 keeps the built-in choice (variant 0, MSGPU_CHUNK, MSGPU_STREAMS,
-// column-major, default panel).
+// 0 column-major), the panel shape and whether trace-log chunks replay a captured HIP
+// graph.  Filled by autotune() or load_tuning(); an entry of 0 keeps the built-in
+// choice (variant 0, MSGPU_CHUNK, MSGPU_STREAMS, column-major, default panel, eager
+// launches).
 struct Tune {
-    int swap = 0, chunk = 0, streams = 0, rm = 0, panel = 0;
+    int swap = 0, chunk = 0, streams = 0, rm = 0, panel = 0, graph = 0;
 };
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, int>, Tune> g_tune;
@@ -288,7 +291,7 @@ static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* p
     const long long ms = (long long)np * np;
     // the panel kernel holds at most 8192 rows of a panel (1024 threads x 8 rows)
     if (np > 8192) throw std::runtime_error("matrices above N = 8192 are not supported by the device LU");
-    HIPCHK(hipMemsetAsync(info, 0, sizeof(int) * nbatch, st));
+    launch_fill_int(info, 0, nbatch, st);
     const double es = sizeof(cx<T>);
     const Tune tn = tune_for(np, prec_of<T>());
     const int swapv = tn.swap, panelv = tn.panel;
@@ -502,6 +505,60 @@ struct DeviceItems {
     }
 };
 
+// ------------------------------------------------------------------ HIP graphs
+// A trace-log chunk (assembly, factorisation, reduction: 3 + 3 Np/64 launches) can be
+// captured once per (stream, shape) and replayed.  The graph owns its workspace: the
+// chunk's items are copied into it before each replay and the results copied out
+// after, so the captured launches always see the same addresses.  Workspaces are
+// kept for the process lifetime, within a total of kGraphBytes; beyond that, chunks
+// are launched eagerly.  The only memory operations inside a graph are kernels.
+template <class T>
+struct ChunkGraph {
+    DBuf<MatItem> items;
+    DBuf<cx<T>> mats;
+    DBuf<int> ipiv, info, pmap;
+    DBuf<cx<double>> out;
+    DBuf<double> stats;
+    hipGraphExec_t exec = nullptr;
+};
+static constexpr size_t kGraphBytes = (size_t)8 << 30;
+static std::mutex g_graph_mu;
+static size_t g_graph_bytes = 0;
+
+template <class T>
+static ChunkGraph<T>* chunk_graph(hipStream_t st, int np, int B, bool rm, const Tune& tn) {
+    using Key = std::tuple<hipStream_t, int, int, int, int, int>;
+    static std::map<Key, std::unique_ptr<ChunkGraph<T>>> cache;
+    const Key key{st, np, B, rm ? 1 : 0, tn.swap, tn.panel};
+    std::lock_guard<std::mutex> g(g_graph_mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second.get();
+    const long long ms = (long long)np * np;
+    const size_t bytes = (size_t)B * ms * sizeof(cx<T>);
+    if (np > 8192 || g_graph_bytes + bytes > kGraphBytes) return nullptr;
+    std::unique_ptr<ChunkGraph<T>> cg(new ChunkGraph<T>());
+    cg->items.alloc(B);
+    cg->mats.alloc((size_t)B * ms);
+    cg->ipiv.alloc((size_t)B * np);
+    cg->info.alloc(B);
+    cg->pmap.alloc((size_t)B * pmap_ints(np, NB));
+    cg->out.alloc(B);
+    cg->stats.alloc(2 * (size_t)B);
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    launch_assemble_items<T>(cg->items.p, np, 0, cg->mats.p, ms, B, st);
+    factor<T>(cg->mats.p, np, B, cg->ipiv.p, cg->info.p, cg->pmap.p, false, rm, st);
+    launch_tracelog<T>(cg->mats.p, 0, cg->items.p, np, ms, cg->ipiv.p, cg->out.p, cg->stats.p, B, st);
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamEndCapture(st, &graph));
+    const hipError_t e = hipGraphInstantiate(&cg->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    g_graph_bytes += bytes;
+    ChunkGraph<T>* raw = cg.get();
+    cache.emplace(key, std::move(cg));
+    return raw;
+}
+
 template <class T>
 static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
                           cx<double>* out, double* stats) {
@@ -529,10 +586,21 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
         const long long ms = (long long)np * np;
         hipStream_t st = S[ln];
         if (c > 0 && (int)c < nlane) HIPCHK(hipStreamWaitEvent(st, stag[c - 1], 0));
+        const Tune tn = tune_for(np, prec_of<T>());
+        ChunkGraph<T>* cg = (tn.graph && !g_prof_on) ? chunk_graph<T>(st, np, B, tn.rm == 1, tn) : nullptr;
+        if (cg) {
+            HIPCHK(hipMemcpyAsync(cg->items.p, D.items.p + ch.start, sizeof(MatItem) * B, hipMemcpyDeviceToDevice, st));
+            if ((int)c + 1 < nlane) HIPCHK(hipEventRecord(stag[c], st));
+            HIPCHK(hipGraphLaunch(cg->exec, st));
+            HIPCHK(hipMemcpyAsync(dout.p + ch.start, cg->out.p, sizeof(cx<double>) * B, hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(dstats.p + 2 * (size_t)ch.start, cg->stats.p, sizeof(double) * 2 * B,
+                                  hipMemcpyDeviceToDevice, st));
+            continue;
+        }
         timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st,
               [&] { launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st); });
-        factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, tune_for(np, prec_of<T>()).rm == 1,
-                  st, ((int)c + 1 < nlane) ? stag[c] : nullptr);
+        factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, tn.rm == 1, st,
+                  ((int)c + 1 < nlane) ? stag[c] : nullptr);
         timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {
             launch_tracelog<T>(mats[ln].p, 0, D.items.p + ch.start, np, ms, ipiv[ln].p, dout.p + ch.start,
                                dstats.p + 2 * (size_t)ch.start, B, st);
@@ -706,6 +774,12 @@ int autotune(int np, int prec, int nitems, double* report) {
         if (ms < best_ms) { best_ms = ms; best = t; }
     }
     {
+        Tune t = best;
+        t.graph = 1;
+        const double ms = trial(t);
+        if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+    {
         std::lock_guard<std::mutex> g(g_tune_mu);
         g_tune[{np, prec}] = best;
     }
@@ -714,6 +788,7 @@ int autotune(int np, int prec, int nitems, double* report) {
         report[0] = best.swap; report[1] = best.chunk; report[2] = best.streams; report[3] = best_ms;
         report[4] = best.rm;
         report[5] = best.panel;
+        report[6] = best.graph;
     }
     return 0;
 }
@@ -722,10 +797,10 @@ int save_tuning(const char* path) {
     FILE* f = fopen(path, "w");
     if (!f) { set_error(std::string("msgpu_tune_save: cannot write ") + path); return -2; }
     std::lock_guard<std::mutex> g(g_tune_mu);
-    fprintf(f, "# np precision swap_variant chunk streams row_major panel_variant\n");
+    fprintf(f, "# np precision swap_variant chunk streams row_major panel_variant graph\n");
     for (auto& kv : g_tune)
-        fprintf(f, "%d %d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.swap, kv.second.chunk,
-                kv.second.streams, kv.second.rm, kv.second.panel);
+        fprintf(f, "%d %d %d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.swap, kv.second.chunk,
+                kv.second.streams, kv.second.rm, kv.second.panel, kv.second.graph);
     fclose(f);
     return 0;
 }
@@ -737,12 +812,12 @@ int load_tuning(const char* path) {
     int n = 0;
     std::lock_guard<std::mutex> g(g_tune_mu);
     while (fgets(line, sizeof line, f)) {
-        int a, b, c, d, e = 0, f6 = 0, g7 = 0;
+        int a, b, c, d, e = 0, f6 = 0, g7 = 0, h8 = 0;
         if (line[0] == '#') continue;
-        const int got = sscanf(line, "%d %d %d %d %d %d %d", &a, &b, &c, &d, &e, &f6, &g7);
+        const int got = sscanf(line, "%d %d %d %d %d %d %d %d", &a, &b, &c, &d, &e, &f6, &g7, &h8);
         if (got >= 4 && c >= 0 && c < swap_variants() && d >= 0 && e >= 0 && e <= NSTREAM_MAX && (f6 == 0 || f6 == 1) &&
-            g7 >= 0 && g7 < kPanelVariants) {
-            g_tune[{a, b}] = Tune{c, d, e, f6, g7};
+            g7 >= 0 && g7 < kPanelVariants && (h8 == 0 || h8 == 1)) {
+            g_tune[{a, b}] = Tune{c, d, e, f6, g7, h8};
             n++;
         }
     }

@@ -767,6 +767,13 @@ void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int
                   hipStream_t st, bool rm, int variant) {
     constexpr bool D = sizeof(T) == 8;
     if (variant < 0 || variant >= kPanelVariants) variant = 0;
+    if (variant == 3) {          // 256 threads: leaves room on the CU for GEMM waves of another stream
+        if (np <= 1024) panel_go<T, 8, 4, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else if (np <= 2048) panel_go<T, 4, 8, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else if (np <= 4096) panel_go<T, 2, 16, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else panel_go<T, 1, 32, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        return;
+    }
     if (np <= 1024) {
         if (variant == 1) {
             if (D) panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
@@ -844,6 +851,16 @@ void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int 
         hipLaunchKernelGGL((trsm_block_kernel<T, 0, true>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
     else
         hipLaunchKernelGGL((trsm_block_kernel<T, 0, false>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
+}
+
+// x[i] = v for i < n (a kernel, so that it may be captured in a HIP graph).
+__global__ void fill_int_kernel(int* __restrict__ x, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+void launch_fill_int(int* x, int v, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, st, x, v, n);
 }
 
 // dst[b][perm[i], c] = src[b][i, c] for i < np, c < ncol (column-major, ld np).

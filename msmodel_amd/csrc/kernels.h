@@ -38,7 +38,7 @@ void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double
 // kern_lu.hip
 // rm: the matrices are stored row-major (see at<> in kern_lu.hip); variant: panel
 // shape index < kPanelVariants (0 = default for the size range)
-constexpr int kPanelVariants = 3;
+constexpr int kPanelVariants = 4;
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
                   hipStream_t st, bool rm = false, int variant = 0);
@@ -63,6 +63,8 @@ void launch_gather_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, i
 template <class T>
 void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int upper, long long slu, long long sb,
                        int nbatch, hipStream_t st, bool rm = false);
+// x[i] = v, i < n
+void launch_fill_int(int* x, int v, int n, hipStream_t st);
 // dst[perm[i], c] = src[i, c]
 template <class T>
 void launch_scatter_rows(cx<T>* dst, const cx<T>* src, const int* perm, int np, int ncol, long long sdst,

@@ -23,7 +23,7 @@ def test_autotune_report_and_results(G):
     dk_before = G.dk_log_det_batch(med, ks[:6], G.FP64)
     rep = G.autotune(300, G.FP64, 64)
     assert 0 <= rep["swap_variant"] < 6 and rep["chunk"] in (128, 256, 512) and rep["streams"] in (1, 2, 4)
-    assert rep["row_major"] in (0, 1) and rep["panel_variant"] in (0, 1, 2)
+    assert rep["row_major"] in (0, 1) and rep["panel_variant"] in (0, 1, 2, 3)
     assert rep["best_ms"] > 0
     after, _ = G.tracelog_batch(med, ks, G.FP64)
     if rep["row_major"] == 0:
@@ -41,7 +41,7 @@ def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
     ks = np.linspace(6, 25, 24) - 0.5j
     ref = None
     for v in range(6):
-        for chunk, streams, rm, pv in ((256, 4, 0, 0), (16, 1, 0, 1), (256, 2, 1, 2), (16, 1, 1, v % 3)):
+        for chunk, streams, rm, pv in ((256, 4, 0, 0), (16, 1, 0, 1), (256, 2, 1, 2), (16, 1, 1, v % 4)):
             path = os.path.join(str(tmp_path), "tune_%d_%d_%d.txt" % (v, chunk, rm))
             with open(path, "w") as f:
                 f.write("384 %d %d %d %d %d %d\n" % (prec, v, chunk, streams, rm, pv))
@@ -60,3 +60,30 @@ def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
     with open(path, "w") as f:
         f.write("384 %d 0 256 4\n" % prec)
     G.tune_load(path)
+
+
+@pytest.mark.parametrize("prec_mode", ["FP64", "FP32"])
+def test_graph_replays_match_eager(G, tmp_path, prec_mode):
+    """Chunks replayed from a captured HIP graph give the eager results bit for bit, on
+    several replays with different inputs and with eager device work in between."""
+    mode = getattr(G, prec_mode)
+    prec = 0 if mode == G.FP64 else 1
+    med = G.Medium(3, 200, G.HARDSPHERE, (0.1,)).fill(1)       # Np = 256
+    other = G.Medium(2, 90, G.HARDSPHERE, (0.2,)).fill(4)
+    sets = [np.linspace(1 + r, 20 - r, 40) - 1j * np.linspace(0, 1, 40)[::-1] * (0.5 + 0.1 * r) for r in range(4)]
+
+    def table(graph):
+        path = os.path.join(str(tmp_path), "g%d.txt" % graph)
+        with open(path, "w") as f:
+            f.write("256 %d 0 16 2 0 0 %d\n" % (prec, graph))
+        assert G.tune_load(path) == 1
+
+    table(0)
+    eager = [G.tracelog_batch(med, ks, mode)[0] for ks in sets]
+    table(1)
+    for rep in range(2):
+        for ks, want in zip(sets, eager):
+            got, _ = G.tracelog_batch(med, ks, mode)
+            assert np.array_equal(got, want), rep
+            G.dk_log_det_batch(other, ks[:3], G.FP64)          # eager work between replays
+    table(0)
