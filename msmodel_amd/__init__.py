@@ -356,10 +356,10 @@ def zgetrs(lu, ipiv, b):
 # ------------------------------------------------------------------ autotuner
 def autotune(n, mode=FP64, nitems=512):
     """Pick the fastest swap + TRSM shape, chunk size and stream count for size n."""
-    rep = np.zeros(7)
+    rep = np.zeros(8)
     _check(lib.msgpu_autotune(n, mode, nitems, _cp(rep)), "msgpu_autotune")
     return {"swap_variant": int(rep[0]), "chunk": int(rep[1]), "streams": int(rep[2]), "best_ms": float(rep[3]),
-            "row_major": int(rep[4]), "panel_variant": int(rep[5]), "graph": int(rep[6])}
+            "row_major": int(rep[4]), "panel_variant": int(rep[5]), "graph": int(rep[6]), "block128": int(rep[7])}
 
 
 def tune_save(path):

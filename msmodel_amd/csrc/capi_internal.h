@@ -80,7 +80,7 @@ int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, i
 // Solve with an existing LAPACK-format LU from the host (zgetrs 'N').
 int lu_solve_host(int n, const cx<double>* lu, int lda, const int* ipiv, int nrhs, cx<double>* b, int ldb);
 // Autotuner (see engine.cpp): report = {swap variant, chunk, streams, best ms, row-major,
-// panel variant, graph}.
+// panel variant, graph, 128-column blocking}.
 int autotune(int np, int prec, int nitems, double* report);
 int save_tuning(const char* path);
 int load_tuning(const char* path);

@@ -257,12 +257,12 @@ static int max_batch(int np, size_t elem, int mats_per_item) {
 // ------------------------------------------------------------------ tuning table
 // Per (padded size, precision): the swap + TRSM tile shape, the chunk size, the
 // stream count, the storage order of the trace-log factorisation (rm: 1 row-major,
-// 0 column-major), the panel shape and whether trace-log chunks replay a captured HIP
-// graph.  Filled by autotune() or load_tuning(); an entry of 0 keeps the built-in
+// 0 column-major), the panel shape, whether trace-log chunks replay a captured HIP
+// graph, and the blocking of the LU (blk2: 128-column blocks, see factor2).  Filled by autotune() or load_tuning(); an entry of 0 keeps the built-in
 // choice (variant 0, MSGPU_CHUNK, MSGPU_STREAMS, column-major, default panel, eager
 // launches).
 struct Tune {
-    int swap = 0, chunk = 0, streams = 0, rm = 0, panel = 0, graph = 0;
+    int swap = 0, chunk = 0, streams = 0, rm = 0, panel = 0, graph = 0, blk2 = 0;
 };
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, int>, Tune> g_tune;
@@ -286,8 +286,24 @@ template <class T> static int prec_of() { return sizeof(T) == 8 ? 0 : 1; }
 // concurrent stream can start its own factorization one panel phase later and the
 // latency-bound panels of one stream run under the GEMMs of the other.
 template <class T>
+static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, bool rm,
+                    hipStream_t st, hipEvent_t after_first_panel);
+
+template <class T>
+static void factor1(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, bool rm,
+                    hipStream_t st, hipEvent_t after_first_panel);
+
+// Blocked LU of a batch (see factor1 / factor2); the tuning table picks the blocking.
+template <class T>
 static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, bool rm,
                    hipStream_t st, hipEvent_t after_first_panel = nullptr) {
+    if (tune_for(np, prec_of<T>()).blk2) factor2<T>(mats, np, nbatch, ipiv, info, pmap, left, rm, st, after_first_panel);
+    else factor1<T>(mats, np, nbatch, ipiv, info, pmap, left, rm, st, after_first_panel);
+}
+
+template <class T>
+static void factor1(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, bool rm,
+                    hipStream_t st, hipEvent_t after_first_panel) {
     const long long ms = (long long)np * np;
     // the panel kernel holds at most 8192 rows of a panel (1024 threads x 8 rows)
     if (np > 8192) throw std::runtime_error("matrices above N = 8192 are not supported by the device LU");
@@ -309,6 +325,70 @@ static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* p
                                          mats + k0 + (size_t)kend * np, rest, rest, NB, np, np, np, ms, ms, ms, nbatch,
                                          st);
                   });
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// Two-level blocking: blocks of 128 columns factored as two 64-column panels, with one
+// trailing update of K = 128 per block instead of two of K = 64 (half the passes over
+// the trailing matrix, whose read and write bound the K = 64 update).  Per block K0
+// (second panel at k1 = K0 + 64, block end ke = K0 + 128):
+//   panel K0; its interchanges and U12 on [k1, ke) (and on [0, K0) with left);
+//   A(k1:, k1:ke) -= L(k1:, K0:k1) U(K0:k1, k1:ke);  panel k1;
+//   its interchanges on [K0, k1) (on [0, k1) with left);
+//   trailing columns [ke, np): interchanges of K0, U12 of K0, interchanges of k1,
+//   A(k1:ke, ke:) -= L(k1:ke, K0:k1) U(K0:k1, ke:), U12 of k1;
+//   A(ke:, ke:) -= L(ke:, K0:ke) U(K0:ke, ke:)   (K = 128).
+// Row-major storage applies each update to the transposed block (see gemm below).
+template <class T>
+static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, bool rm,
+                    hipStream_t st, hipEvent_t after_first_panel) {
+    const long long ms = (long long)np * np;
+    if (np > 8192) throw std::runtime_error("matrices above N = 8192 are not supported by the device LU");
+    launch_fill_int(info, 0, nbatch, st);
+    const double es = sizeof(cx<T>);
+    const Tune tn = tune_for(np, prec_of<T>());
+    // M(r0:r0+m, c0:c0+n) -= M(r0:, p0:p0+k) M(p0:, c0:).  Row-major storage read column-major
+    // is S = M^T, where this is S(c0:, r0:) -= S(c0:, p0:) S(p0:, r0:): the transposed block.
+    auto gemm = [&](int r0, int c0, int p0, int m, int n, int k) {
+        if (m <= 0 || n <= 0) return;
+        const int R = rm ? c0 : r0, Cc = rm ? r0 : c0;
+        timed(2, 8.0 * nbatch * (double)m * n * k, es * nbatch * (2.0 * m * n + (double)(m + n) * k), st, [&] {
+            launch_gemm_sub<T>(mats + R + (size_t)Cc * np, mats + R + (size_t)p0 * np, mats + p0 + (size_t)Cc * np,
+                               rm ? n : m, rm ? m : n, k, np, np, np, ms, ms, ms, nbatch, st);
+        });
+    };
+    auto panel = [&](int k0) {
+        timed(0, 8.0 * nbatch * ((double)(np - k0) * NB * NB / 2), es * nbatch * 2.0 * (np - k0) * NB, st,
+              [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st, rm, tn.panel); });
+    };
+    auto swap = [&](int k0, int lb, int rb, int re, int do_swap, int do_trsm) {
+        const double cols = (double)std::max(0, re - rb) + std::max(0, k0 - lb);
+        timed(1, do_trsm ? 8.0 * nbatch * (double)NB * NB / 2 * std::max(0, re - rb) : 0.0,
+              es * nbatch * 2.0 * NB * cols, st, [&] {
+                  launch_swap_range<T>(mats, np, ms, pmap, k0, NB, lb, rb, re, do_swap, do_trsm, nbatch, st, tn.swap,
+                                       rm);
+              });
+    };
+    for (int K0 = 0; K0 < np; K0 += 2 * NB) {
+        const int k1 = K0 + NB, ke = K0 + 2 * NB;
+        panel(K0);
+        if (K0 == 0 && after_first_panel) HIPCHK(hipEventRecord(after_first_panel, st));
+        if (k1 >= np) {                               // a last, single panel
+            swap(K0, left ? 0 : K0, np, np, 1, 0);
+            break;
+        }
+        swap(K0, left ? 0 : K0, k1, ke, 1, 1);
+        gemm(k1, k1, K0, np - k1, NB, NB);
+        panel(k1);
+        swap(k1, left ? 0 : K0, 0, 0, 1, 0);
+        if (ke < np) {
+            swap(K0, K0, ke, np, 1, 1);
+            swap(k1, k1, ke, np, 1, 0);
+            gemm(k1, ke, K0, NB, np - ke, NB);
+            swap(k1, k1, ke, np, 0, 1);
+            gemm(ke, ke, K0, np - ke, np - ke, 2 * NB);
+        }
     }
     HIPCHK(hipGetLastError());
 }
@@ -527,9 +607,9 @@ static size_t g_graph_bytes = 0;
 
 template <class T>
 static ChunkGraph<T>* chunk_graph(hipStream_t st, int np, int B, bool rm, const Tune& tn) {
-    using Key = std::tuple<hipStream_t, int, int, int, int, int>;
+    using Key = std::tuple<hipStream_t, int, int, int, int, int, int>;
     static std::map<Key, std::unique_ptr<ChunkGraph<T>>> cache;
-    const Key key{st, np, B, rm ? 1 : 0, tn.swap, tn.panel};
+    const Key key{st, np, B, rm ? 1 : 0, tn.swap, tn.panel, tn.blk2};
     std::lock_guard<std::mutex> g(g_graph_mu);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second.get();
@@ -747,14 +827,16 @@ int autotune(int np, int prec, int nitems, double* report) {
     best.chunk = chunk_cap();
     best.streams = default_streams();
     double best_ms = INFINITY;
-    for (int rm = 0; rm < 2; rm++)
-        for (int v = 0; v < swap_variants(); v++) {
-            Tune t = best;
-            t.swap = v;
-            t.rm = rm;
-            const double ms = trial(t);
-            if (ms < best_ms) { best_ms = ms; best = t; }
-        }
+    for (int b2 = 0; b2 < 2; b2++)
+        for (int rm = 0; rm < 2; rm++)
+            for (int v = 0; v < swap_variants(); v++) {
+                Tune t = best;
+                t.swap = v;
+                t.rm = rm;
+                t.blk2 = b2;
+                const double ms = trial(t);
+                if (ms < best_ms) { best_ms = ms; best = t; }
+            }
     for (int pv = 1; pv < kPanelVariants; pv++) {
         Tune t = best;
         t.panel = pv;
@@ -789,6 +871,7 @@ int autotune(int np, int prec, int nitems, double* report) {
         report[4] = best.rm;
         report[5] = best.panel;
         report[6] = best.graph;
+        report[7] = best.blk2;
     }
     return 0;
 }
@@ -797,10 +880,10 @@ int save_tuning(const char* path) {
     FILE* f = fopen(path, "w");
     if (!f) { set_error(std::string("msgpu_tune_save: cannot write ") + path); return -2; }
     std::lock_guard<std::mutex> g(g_tune_mu);
-    fprintf(f, "# np precision swap_variant chunk streams row_major panel_variant graph\n");
+    fprintf(f, "# np precision swap_variant chunk streams row_major panel_variant graph block128\n");
     for (auto& kv : g_tune)
-        fprintf(f, "%d %d %d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.swap, kv.second.chunk,
-                kv.second.streams, kv.second.rm, kv.second.panel, kv.second.graph);
+        fprintf(f, "%d %d %d %d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.swap, kv.second.chunk,
+                kv.second.streams, kv.second.rm, kv.second.panel, kv.second.graph, kv.second.blk2);
     fclose(f);
     return 0;
 }
@@ -812,12 +895,12 @@ int load_tuning(const char* path) {
     int n = 0;
     std::lock_guard<std::mutex> g(g_tune_mu);
     while (fgets(line, sizeof line, f)) {
-        int a, b, c, d, e = 0, f6 = 0, g7 = 0, h8 = 0;
+        int a, b, c, d, e = 0, f6 = 0, g7 = 0, h8 = 0, i9 = 0;
         if (line[0] == '#') continue;
-        const int got = sscanf(line, "%d %d %d %d %d %d %d %d", &a, &b, &c, &d, &e, &f6, &g7, &h8);
+        const int got = sscanf(line, "%d %d %d %d %d %d %d %d %d", &a, &b, &c, &d, &e, &f6, &g7, &h8, &i9);
         if (got >= 4 && c >= 0 && c < swap_variants() && d >= 0 && e >= 0 && e <= NSTREAM_MAX && (f6 == 0 || f6 == 1) &&
-            g7 >= 0 && g7 < kPanelVariants && (h8 == 0 || h8 == 1)) {
-            g_tune[{a, b}] = Tune{c, d, e, f6, g7, h8};
+            g7 >= 0 && g7 < kPanelVariants && (h8 == 0 || h8 == 1) && (i9 == 0 || i9 == 1)) {
+            g_tune[{a, b}] = Tune{c, d, e, f6, g7, h8, i9};
             n++;
         }
     }

@@ -342,7 +342,8 @@ __device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], const cx<T> (*sL)[6
 template <class T, int TW, int RW, bool RM>
 __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, int np, long long mstride,
                                                         const int* __restrict__ pmap_all, int k0, int nb,
-                                                        int nleft_t, int do_trsm) {
+                                                        SwapRange sr) {
+    const int nleft_t = sr.nleft_t;
     constexpr int U = (128 * RW + 255) / 256;
     __shared__ int s_dst[128], s_src[128];
     __shared__ cx<T> sL[64][65];
@@ -353,8 +354,9 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
     const int kend = min(k0 + nb, np);
     const int tid = threadIdx.x;
     const bool left = (int)blockIdx.x < nleft_t;
-    const int col0 = left ? blockIdx.x * TW : kend + (blockIdx.x - nleft_t) * TW;
-    const int colend = left ? min(col0 + TW, k0) : min(col0 + TW, np);
+    const int col0 = left ? sr.lb + blockIdx.x * TW : sr.rb + (blockIdx.x - nleft_t) * TW;
+    const int colend = left ? min(col0 + TW, k0) : min(col0 + TW, sr.re);
+    const int do_trsm = sr.do_trsm;
     const int ncol = colend - col0;
     const int cnt = pmap[0];
     for (int e = tid; e < cnt; e += 256) { s_dst[e] = pmap[1 + e]; s_src[e] = pmap[1 + 2 * nb + e]; }
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
         }
     }
     __syncthreads();
-    if (cnt > 0) {
+    if (cnt > 0 && sr.do_swap) {
         for (int cb = 0; cb < ncol; cb += RW) {
             const int w = min(RW, ncol - cb), pairs = cnt * w;
             cx<T> tmp[U];
@@ -421,22 +423,22 @@ constexpr int kNumSwapShapes = sizeof(kSwapShapes) / sizeof(kSwapShapes[0]);
 
 template <class T, bool RM>
 static void swap_go_rm(int variant, dim3 grid, hipStream_t st, cx<T>* A, int np, long long mstride, const int* pmap,
-                       int k0, int nb, int nleft_t, int do_trsm) {
+                       int k0, int nb, SwapRange sr) {
     switch (variant) {
-        case 1: hipLaunchKernelGGL((swap_trsm_kernel<T, 128, 16, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
-        case 2: hipLaunchKernelGGL((swap_trsm_kernel<T, 64, 16, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
-        case 3: hipLaunchKernelGGL((swap_trsm_kernel<T, 64, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
-        case 4: hipLaunchKernelGGL((swap_trsm_kernel<T, 128, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
-        case 5: hipLaunchKernelGGL((swap_trsm_kernel<T, 256, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
-        default: hipLaunchKernelGGL((swap_trsm_kernel<T, 256, 8, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm); break;
+        case 1: hipLaunchKernelGGL((swap_trsm_kernel<T, 128, 16, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, sr); break;
+        case 2: hipLaunchKernelGGL((swap_trsm_kernel<T, 64, 16, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, sr); break;
+        case 3: hipLaunchKernelGGL((swap_trsm_kernel<T, 64, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, sr); break;
+        case 4: hipLaunchKernelGGL((swap_trsm_kernel<T, 128, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, sr); break;
+        case 5: hipLaunchKernelGGL((swap_trsm_kernel<T, 256, 32, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, sr); break;
+        default: hipLaunchKernelGGL((swap_trsm_kernel<T, 256, 8, RM>), grid, dim3(256), 0, st, A, np, mstride, pmap, k0, nb, sr); break;
     }
 }
 
 template <class T>
 static void swap_go(int variant, dim3 grid, hipStream_t st, cx<T>* A, int np, long long mstride, const int* pmap,
-                    int k0, int nb, int nleft_t, int do_trsm, bool rm) {
-    if (rm) swap_go_rm<T, true>(variant, grid, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm);
-    else swap_go_rm<T, false>(variant, grid, st, A, np, mstride, pmap, k0, nb, nleft_t, do_trsm);
+                    int k0, int nb, SwapRange sr, bool rm) {
+    if (rm) swap_go_rm<T, true>(variant, grid, st, A, np, mstride, pmap, k0, nb, sr);
+    else swap_go_rm<T, false>(variant, grid, st, A, np, mstride, pmap, k0, nb, sr);
 }
 
 int swap_variants() { return kNumSwapShapes; }
@@ -502,6 +504,12 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(cx<T>* C, const cx<T>* A,
         }
     const int kq = lane >> 4, rot = kq & 1;
     const int ar = mfma<T>::arow(lane & 15);
+    // A fragment of [[Ar, -Ai], [Ai, Ar]]: element (rr, p) of the rotation is
+    // -lA[p][rr] (rot = 0), -lA[p][rr-1] (rot = 1, rr odd) or +lA[p][rr+1] (rot = 1, rr
+    // even).  rr's parity is ar's, so offset and sign are fixed per lane: one LDS read
+    // and no branch per fragment.
+    const int aoff = rot ? ((ar & 1) ? -1 : 1) : 0;
+    const T asgn = (rot && !(ar & 1)) ? T(1) : T(-1);
     for (int kc0 = 0; kc0 < kk; kc0 += KC) {
         __syncthreads();
 #pragma unroll
@@ -521,14 +529,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(cx<T>* C, const cx<T>* A,
             const int p = (ks >> 1) + (kq >> 1);
             T av[4], bv[2];
 #pragma unroll
-            for (int rb = 0; rb < 4; rb++) {
-                const int rr = wr * 64 + rb * 16 + ar;
-                const int comp = rr & 1;
-                T v;
-                if (!rot) v = -lA[p][rr];
-                else v = comp ? -lA[p][rr - 1] : lA[p][rr + 1];
-                av[rb] = v;
-            }
+            for (int rb = 0; rb < 4; rb++) av[rb] = asgn * lA[p][wr * 64 + rb * 16 + ar + aoff];
 #pragma unroll
             for (int cb = 0; cb < 2; cb++) bv[cb] = lB[wc * 32 + cb * 16 + (lane & 15)][ks + kq];
 #pragma unroll
@@ -802,13 +803,25 @@ void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int
 template <class T>
 void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,
                       int nbatch, hipStream_t st, int variant, bool rm) {
+    const int kend = std::min(k0 + nb, np);
+    launch_swap_range<T>(A, np, mstride, pmap, k0, nb, apply_left ? 0 : k0, kend, np, 1, 1, nbatch, st, variant, rm);
+}
+
+template <class T>
+void launch_swap_range(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int lb, int rb, int re,
+                       int do_swap, int do_trsm, int nbatch, hipStream_t st, int variant, bool rm) {
     if (variant < 0 || variant >= kNumSwapShapes) variant = 0;
     const int tw = kSwapShapes[variant].tw;
-    const int kend = std::min(k0 + nb, np);
-    const int nleft_t = apply_left ? (k0 + tw - 1) / tw : 0;
-    const int nright_t = (np - kend + tw - 1) / tw;
-    if (nleft_t + nright_t == 0) return;
-    swap_go<T>(variant, dim3(nleft_t + nright_t, nbatch), st, A, np, mstride, pmap, k0, nb, nleft_t, 1, rm);
+    SwapRange sr;
+    sr.lb = lb;
+    sr.nleft_t = (do_swap && lb < k0) ? (k0 - lb + tw - 1) / tw : 0;
+    sr.rb = rb;
+    sr.re = re;
+    sr.do_swap = do_swap;
+    sr.do_trsm = do_trsm;
+    const int nright_t = re > rb ? (re - rb + tw - 1) / tw : 0;
+    if (sr.nleft_t + nright_t == 0) return;
+    swap_go<T>(variant, dim3(sr.nleft_t + nright_t, nbatch), st, A, np, mstride, pmap, k0, nb, sr, rm);
 }
 
 template <class T>
@@ -916,9 +929,7 @@ void launch_apply_map(cx<T>* X, int np, int ncol, const int* pmap, int nb, int q
 template <class T>
 void launch_swap_left(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int nbatch,
                       hipStream_t st) {
-    const int nleft_t = (k0 + 255) / 256;
-    if (nleft_t == 0) return;
-    swap_go<T>(0, dim3(nleft_t, nbatch), st, A, np, mstride, pmap, k0, nb, nleft_t, 0, false);
+    launch_swap_range<T>(A, np, mstride, pmap, k0, nb, 0, 0, 0, 1, 0, nbatch, st, 0, false);
 }
 
 template <class T>
@@ -931,6 +942,8 @@ void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st) {
 #define MSG_INST(T)                                                                                                   \
     template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t, bool, int);   \
     template void launch_swap_trsm<T>(cx<T>*, int, long long, const int*, int, int, int, int, hipStream_t, int, bool);\
+    template void launch_swap_range<T>(cx<T>*, int, long long, const int*, int, int, int, int, int, int, int, int,    \
+                                       hipStream_t, int, bool);                                                        \
     template void launch_gemm_sub<T>(cx<T>*, const cx<T>*, const cx<T>*, int, int, int, int, int, int, long long,      \
                                      long long, long long, int, hipStream_t);                                          \
     template void launch_tracelog<T>(const cx<T>*, int, const MatItem*, int, long long, const int*, cx<double>*,     \

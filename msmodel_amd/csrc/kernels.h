@@ -47,6 +47,14 @@ template <class T>
 void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,
                       int nbatch, hipStream_t st, int variant = 0, bool rm = false);
 int swap_variants();
+// Row map of panel k0 on explicit column ranges: left tiles cover [lb, k0) (swap only),
+// right tiles [rb, re) (swap if do_swap, then U12 = L11^-1 A12 if do_trsm).
+struct SwapRange {
+    int lb, nleft_t, rb, re, do_swap, do_trsm;
+};
+template <class T>
+void launch_swap_range(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int lb, int rb, int re,
+                       int do_swap, int do_trsm, int nbatch, hipStream_t st, int variant = 0, bool rm = false);
 template <class T>
 void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int k, int lda, int ldb, int ldc,
                      long long sC, long long sA, long long sB, int nbatch, hipStream_t st);

@@ -53,10 +53,10 @@ def test_tuning_table_round_trip(tmp_path):
     """The tuning table loads and saves as text without a device."""
     import msmodel_amd as G
     src = tmp_path / "in.txt"
-    src.write_text("# np precision swap_variant chunk streams row_major panel_variant graph\n"
-                   "1024 0 2 512 2 1 2 1\n1024 1 1 256 4\n9999 0 99 1 1 0 0 0\n")
+    src.write_text("# np precision swap_variant chunk streams row_major panel_variant graph block128\n"
+                   "1024 0 2 512 2 1 2 1 1\n1024 1 1 256 4\n9999 0 99 1 1 0 0 0 0\n")
     assert G.tune_load(str(src)) == 2          # the out-of-range variant is skipped
     dst = tmp_path / "out.txt"
     G.tune_save(str(dst))
     lines = dst.read_text().split("\n")
-    assert "1024 0 2 512 2 1 2 1" in lines and "1024 1 1 256 4 0 0 0" in lines
+    assert "1024 0 2 512 2 1 2 1 1" in lines and "1024 1 1 256 4 0 0 0 0" in lines

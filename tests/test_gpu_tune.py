@@ -1,6 +1,8 @@
-"""The autotuner and the tuning table: every tunable (swap + TRSM tile shape, chunk
-size, stream count) only moves data or changes the launch shape, so results are
-bit-identical whatever the table holds."""
+"""The autotuner and the tuning table.  The swap + TRSM tile shape, panel shape, chunk
+size, stream count, graph replay and the 128-column blocking only move data, change
+launch shapes or keep the accumulation order, so results are bit-identical whatever
+the table holds; the row-major factorisation rounds differently and is held to the
+parity tolerance."""
 import os
 
 import numpy as np
@@ -23,7 +25,7 @@ def test_autotune_report_and_results(G):
     dk_before = G.dk_log_det_batch(med, ks[:6], G.FP64)
     rep = G.autotune(300, G.FP64, 64)
     assert 0 <= rep["swap_variant"] < 6 and rep["chunk"] in (128, 256, 512) and rep["streams"] in (1, 2, 4)
-    assert rep["row_major"] in (0, 1) and rep["panel_variant"] in (0, 1, 2, 3)
+    assert rep["row_major"] in (0, 1) and rep["panel_variant"] in (0, 1, 2, 3) and rep["block128"] in (0, 1)
     assert rep["best_ms"] > 0
     after, _ = G.tracelog_batch(med, ks, G.FP64)
     if rep["row_major"] == 0:
@@ -41,16 +43,17 @@ def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
     ks = np.linspace(6, 25, 24) - 0.5j
     ref = None
     for v in range(6):
-        for chunk, streams, rm, pv in ((256, 4, 0, 0), (16, 1, 0, 1), (256, 2, 1, 2), (16, 1, 1, v % 4)):
-            path = os.path.join(str(tmp_path), "tune_%d_%d_%d.txt" % (v, chunk, rm))
+        for chunk, streams, rm, pv, b2 in ((256, 4, 0, 0, 0), (16, 1, 0, 1, 1), (256, 2, 1, 2, 0), (16, 1, 1, v % 4, 1),
+                                           (128, 2, 0, 3, 1)):
+            path = os.path.join(str(tmp_path), "tune_%d_%d_%d_%d.txt" % (v, chunk, rm, b2))
             with open(path, "w") as f:
-                f.write("384 %d %d %d %d %d %d\n" % (prec, v, chunk, streams, rm, pv))
+                f.write("384 %d %d %d %d %d %d 0 %d\n" % (prec, v, chunk, streams, rm, pv, b2))
             assert G.tune_load(path) == 1
             out, _ = G.tracelog_batch(med, ks, mode)
             if ref is None:
                 ref = out
-            if rm == 0:
-                assert np.array_equal(out, ref), (v, chunk, streams)
+            if rm == 0:     # the 128-column blocking accumulates in the same order: exact
+                assert np.array_equal(out, ref), (v, chunk, streams, b2)
             else:   # the row-major factorisation rounds differently: parity tolerance
                 tol = 1e-10 if prec == 0 else 1e-4
                 assert np.abs(out.real - ref.real).max() <= tol * max(1, np.abs(ref.real).max())
