@@ -473,7 +473,18 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(cx<T>* C, const cx<T>* A,
     constexpr int LU = KC * GT / 256;     // complex elements per thread per operand and chunk
     // prefetch registers for the next K chunk
     cx<T> pa[LU], pb[LU];
+    // interior tiles (the common case) load without per-element bounds tests
+    const bool full_mn = row0 + GT <= m && col0 + GT <= n;
     auto fetch = [&](int kc0) {
+        if (full_mn && kc0 + KC <= kk) {
+#pragma unroll
+            for (int u = 0; u < LU; u++) {
+                const int e = tid + 256 * u;
+                pa[u] = A[(size_t)(row0 + e % GT) + (size_t)(kc0 + e / GT) * lda];
+                pb[u] = B[(size_t)(kc0 + e % KC) + (size_t)(col0 + e / KC) * ldb];
+            }
+            return;
+        }
 #pragma unroll
         for (int u = 0; u < LU; u++) {
             const int e = tid + 256 * u;
