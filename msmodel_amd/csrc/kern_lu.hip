@@ -779,6 +779,14 @@ void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int
                   hipStream_t st, bool rm, int variant) {
     constexpr bool D = sizeof(T) == 8;
     if (variant < 0 || variant >= kPanelVariants) variant = 0;
+    if (variant == 4) {          // 16-column sub-panels: half the passes over the panel's right part
+        if (np <= 1024) {
+            if (D) panel_go<T, 16, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+            else panel_go<T, 16, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+            return;
+        }
+        variant = 0;
+    }
     if (variant == 3) {          // 256 threads: leaves room on the CU for GEMM waves of another stream
         if (np <= 1024) panel_go<T, 8, 4, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
         else if (np <= 2048) panel_go<T, 4, 8, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);

@@ -25,7 +25,7 @@ def test_autotune_report_and_results(G):
     dk_before = G.dk_log_det_batch(med, ks[:6], G.FP64)
     rep = G.autotune(300, G.FP64, 64)
     assert 0 <= rep["swap_variant"] < 6 and rep["chunk"] in (128, 256, 512) and rep["streams"] in (1, 2, 4)
-    assert rep["row_major"] in (0, 1) and rep["panel_variant"] in (0, 1, 2, 3) and rep["block128"] in (0, 1)
+    assert rep["row_major"] in (0, 1) and rep["panel_variant"] in (0, 1, 2, 3, 4) and rep["block128"] in (0, 1)
     assert rep["best_ms"] > 0
     after, _ = G.tracelog_batch(med, ks, G.FP64)
     if rep["row_major"] == 0:
@@ -43,8 +43,8 @@ def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
     ks = np.linspace(6, 25, 24) - 0.5j
     ref = None
     for v in range(6):
-        for chunk, streams, rm, pv, b2 in ((256, 4, 0, 0, 0), (16, 1, 0, 1, 1), (256, 2, 1, 2, 0), (16, 1, 1, v % 4, 1),
-                                           (128, 2, 0, 3, 1)):
+        for chunk, streams, rm, pv, b2 in ((256, 4, 0, 0, 0), (16, 1, 0, 1, 1), (256, 2, 1, 2, 0), (16, 1, 1, v % 5, 1),
+                                           (128, 2, 0, 3, 1), (64, 1, 0, 4, 0)):
             path = os.path.join(str(tmp_path), "tune_%d_%d_%d_%d.txt" % (v, chunk, rm, b2))
             with open(path, "w") as f:
                 f.write("384 %d %d %d %d %d %d 0 %d\n" % (prec, v, chunk, streams, rm, pv, b2))
