@@ -446,7 +446,13 @@ int swap_variants() { return kNumSwapShapes; }
 // ------------------------------------------------------------------ GEMM: C -= A B
 constexpr int GT = 64;     // complex tile edge (rows and columns)
 constexpr int KC = 16;     // complex K chunk staged in LDS
-constexpr int PADA = 2, PADB = 2;
+#ifndef MSG_GEMM_PADA
+#define MSG_GEMM_PADA 2
+#endif
+#ifndef MSG_GEMM_PADB
+#define MSG_GEMM_PADB 2
+#endif
+constexpr int PADA = MSG_GEMM_PADA, PADB = MSG_GEMM_PADB;   // LDS row padding (elements of T)
 
 template <class T>
 __global__ __launch_bounds__(256) void gemm_sub_kernel(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n,
