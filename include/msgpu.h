@@ -125,12 +125,12 @@ int msgpu_profile_begin(void);
 int msgpu_profile_end(double* ms, double* flops, double* bytes, int* count, int ncat);
 
 /* Autotuner (no counterpart in the reference): times the batched trace-log of nitems
- * matrices of size N (synthetic d=3 unit-density medium) for each storage order of the
- * factorisation, swap + TRSM tile shape, panel shape, chunk size, stream count and
- * HIP-graph replay of the chunk pipeline, and keeps the fastest for N's padded size and
+ * matrices of size N (synthetic d=3 unit-density medium) for each blocking (64 or 128
+ * columns) and storage order of the factorisation, swap + TRSM tile shape, panel shape,
+ * chunk size, stream count and HIP-graph replay of the chunk pipeline, and keeps the fastest for N's padded size and
  * the mode's precision (complex128 for MSGPU_FP64, complex64 otherwise); later calls
  * use it.  report (may be NULL) = {swap variant, chunk, streams, best ms, row-major,
- * panel variant, graph replay}.
+ * panel variant, graph replay, 128-column blocking}.
  * msgpu_tune_save/_load write/read the table as text; MSGPU_TUNE_FILE is loaded at
  * initialisation.  _load returns the number of entries read. */
 int msgpu_autotune(int n, int mode, int nitems, double* report);

@@ -837,17 +837,18 @@ int autotune(int np, int prec, int nitems, double* report) {
                 const double ms = trial(t);
                 if (ms < best_ms) { best_ms = ms; best = t; }
             }
-    for (int pv = 1; pv < kPanelVariants; pv++) {
-        Tune t = best;
-        t.panel = pv;
-        const double ms = trial(t);
-        if (ms < best_ms) { best_ms = ms; best = t; }
-    }
-    for (int c : {128, 256, 512}) {
-        Tune t = best;
-        t.chunk = c;
-        const double ms = trial(t);
-        if (ms < best_ms) { best_ms = ms; best = t; }
+    {   // panel shape and chunk size together: smaller panel workgroups pay off with more
+        // matrices per launch (several per CU)
+        const Tune base = best;
+        for (int pv = 0; pv < kPanelVariants; pv++)
+            for (int c : {128, 256, 512}) {
+                Tune t = base;
+                t.panel = pv;
+                t.chunk = c;
+                if (pv == base.panel && c == base.chunk) continue;
+                const double ms = trial(t);
+                if (ms < best_ms) { best_ms = ms; best = t; }
+            }
     }
     for (int ns : {1, 2, 4}) {
         Tune t = best;

@@ -71,6 +71,12 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
     __shared__ cx<T> s_U[S][64];
     __shared__ int s_dst[128], s_src[128], s_cnt;
     __shared__ int s_sd[2 * S], s_ss[2 * S], s_scnt;
+    // L rows of the current sub-panel for the LDS-broadcast rank-S update: for the
+    // 1024-thread shapes (one workgroup per CU anyway), when they fit beside s_slot (4 B
+    // per row, dynamic) and the arrays above within the 160 KB of LDS.  Smaller shapes
+    // keep their LDS small so that several matrices share a CU.
+    constexpr bool LDSU = PT == 1024 && (size_t)PT * R * (S * sizeof(cx<T>) + 4) <= 139264;
+    __shared__ T sLre[LDSU ? S : 1][LDSU ? PT * R : 1], sLim[LDSU ? S : 1][LDSU ? PT * R : 1];
 
     const int b = blockIdx.x;
     cx<T>* M = A + (size_t)b * mstride;
@@ -247,8 +253,35 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
                 for (int i = 0; i < S; i++) M[at<RM>(c0 + i, col, np)] = s_U[i][tid];
             }
             __syncthreads();
-            // rank-S update of rows below the sub-panel, WR columns per round
-            constexpr int WR = sizeof(T) == 8 ? 4 : 8;
+            if constexpr (LDSU) {
+                // rank-S update with the sub-panel's L rows broadcast through LDS: every thread
+                // updates whole (row, column) elements in storage order, so the loads of
+                // consecutive elements are independent and coalesced
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int gl = tid + PT * r;
+#pragma unroll
+                    for (int m = 0; m < S; m++) { sLre[m][gl] = x[r][m].x; sLim[m][gl] = x[r][m].y; }
+                }
+                __syncthreads();
+                const int nrows = np - c0 - S;
+                const int tot = nrows * nright;
+#pragma unroll 2
+                for (int e = tid; e < tot; e += PT) {
+                    const int ri = RM ? e / nright : e % nrows, ci = RM ? e % nright : e / nrows;
+                    const size_t at_e = at<RM>(c0 + S + ri, c0 + S + ci, np);
+                    cx<T> a = M[at_e];
+#pragma unroll
+                    for (int m = 0; m < S; m++) a = a - cx<T>{sLre[m][S + ri], sLim[m][S + ri]} * s_U[m][ci];
+                    M[at_e] = a;
+                }
+                __syncthreads();
+                continue;
+            }
+            // rank-S update of rows below the sub-panel, WR columns per round (the loads of a
+            // round are in flight together; shapes of <= 512 threads with few register rows
+            // have the registers for 16)
+            constexpr int WR = (PT <= 512 && R * S * (int)sizeof(T) <= 128) ? 16 : (sizeof(T) == 8 ? 4 : 8);
             for (int w0 = 0; w0 < nright; w0 += WR) {
 #pragma unroll
                 for (int r = 0; r < R; r++) {
@@ -445,7 +478,10 @@ int swap_variants() { return kNumSwapShapes; }
 
 // ------------------------------------------------------------------ GEMM: C -= A B
 constexpr int GT = 64;     // complex tile edge (rows and columns)
-constexpr int KC = 16;     // complex K chunk staged in LDS
+#ifndef MSG_GEMM_KC
+#define MSG_GEMM_KC 16
+#endif
+constexpr int KC = MSG_GEMM_KC;     // complex K chunk staged in LDS
 #ifndef MSG_GEMM_PADA
 #define MSG_GEMM_PADA 2
 #endif
