@@ -181,39 +181,61 @@ def uneven_search(G, nmed, per, nmin, nmax):
             "sample": "%d media x %d starts (the full search runs 10000 starts)" % (nmed, per)}
 
 
-def small_map(G, steps):
-    """The reference's CPU-runnable map: d=3, N=200 in a ball of radius 1 (seed 1),
-    alpha=0.1, 128x64 k over Re[1,20] x Im[-1,0], complex128; with the chunk pipeline
-    launched eagerly and replayed from captured HIP graphs (tuned launch shapes)."""
+def tune_entry(G, np_, prec):
+    """The tuning-table entry for padded size np_ and precision prec (0 complex128, 1
+    complex64) as a dict, or None when the table has none."""
     import tempfile
-    med = G.Medium(3, 200, G.HARDSPHERE, (0.1,)).fill(1, radius=1.0)
-    ks = kgrid(128, 64, 1.0, 20.0, -1.0, 0.0)
-    tuned = G.autotune(200, G.FP64, 1024)
-    out = {"tuned": tuned}
+    keys = ("swap_variant", "chunk", "streams", "row_major", "panel_variant", "graph", "block128")
     with tempfile.TemporaryDirectory() as tmp:
-        for graph in (0, 1):
-            path = os.path.join(tmp, "t.txt")
-            with open(path, "w") as f:
-                f.write("256 0 %d %d %d %d %d %d\n" % (tuned["swap_variant"], tuned["chunk"], tuned["streams"],
-                                                        tuned["row_major"], tuned["panel_variant"], graph))
-            G.tune_load(path)
-            G.tracelog_batch(med, ks, G.FP64)
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                res, _ = G.tracelog_batch(med, ks, G.FP64)
-            out["graph" if graph else "eager"] = steps * ks.size / (time.perf_counter() - t0)
-            out["tracelog_%d" % graph] = res
-    same = bool(np.array_equal(out.pop("tracelog_0"), out.pop("tracelog_1")))
-    # restore the tuned choice
+        path = os.path.join(tmp, "t.txt")
+        G.tune_save(path)
+        for line in open(path):
+            f = line.split()
+            if line.startswith("#") or len(f) < 9:
+                continue
+            if int(f[0]) == np_ and int(f[1]) == prec:
+                return dict(zip(keys, map(int, f[2:9])))
+    return None
+
+
+def set_entry(G, np_, prec, e):
+    import tempfile
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "t.txt")
         with open(path, "w") as f:
-            f.write("256 0 %d %d %d %d %d %d\n" % (tuned["swap_variant"], tuned["chunk"], tuned["streams"],
-                                                    tuned["row_major"], tuned["panel_variant"], tuned["graph"]))
+            f.write("%d %d %d %d %d %d %d %d %d\n" % (np_, prec, e["swap_variant"], e["chunk"], e["streams"],
+                                                      e["row_major"], e["panel_variant"], e["graph"], e["block128"]))
         G.tune_load(path)
+
+
+def small_map(G, steps, tune):
+    """The reference's CPU-runnable map: d=3, N=200 in a ball of radius 1 (seed 1),
+    alpha=0.1, 128x64 k over Re[1,20] x Im[-1,0], complex128; with the chunk pipeline
+    launched eagerly and replayed from captured HIP graphs (tuned launch shapes)."""
+    med = G.Medium(3, 200, G.HARDSPHERE, (0.1,)).fill(1, radius=1.0)
+    ks = kgrid(128, 64, 1.0, 20.0, -1.0, 0.0)
+    tuned = tune_entry(G, 256, 0)
+    if tune or tuned is None:
+        G.autotune(200, G.FP64, 1024)
+        tuned = tune_entry(G, 256, 0)
+    out = {}
+    for graph in (0, 1):
+        set_entry(G, 256, 0, dict(tuned, graph=graph))
+        G.tracelog_batch(med, ks, G.FP64)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res, _ = G.tracelog_batch(med, ks, G.FP64)
+        out["graph" if graph else "eager"] = steps * ks.size / (time.perf_counter() - t0)
+        out["tracelog_%d" % graph] = res
+    same = bool(np.array_equal(out.pop("tracelog_0"), out.pop("tracelog_1")))
+    set_entry(G, 256, 0, tuned)
     return {"workload": "tracelog map d=3 N=200 ball R=1 seed 1 hardsphere 0.1, 128x64 k over Re[1,20] x Im[-1,0]",
             "evals_per_s_eager": out["eager"], "evals_per_s_graph": out["graph"], "graph_equals_eager": same,
-            "autotune": tuned}
+            "launch_shapes": tuned}
+
+
+def progress(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def wavefunction_case(G):
@@ -251,8 +273,11 @@ def main():
     ap.add_argument("--uneven", type=int, default=64, help="media of the uneven resonance search (0: skip)")
     ap.add_argument("--uneven-starts", type=int, default=4, help="starts per medium of the uneven search")
     ap.add_argument("--no-wave", action="store_true", help="skip the solve + 1024^2 psi case")
-    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shapes (no autotuning)")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner for N before timing")
+    ap.add_argument("--tune-file", default="", help="with --tune: where to save the tuning table")
+    ap.add_argument("--tune-sizes", default="200,500,2000", help="with --tune: further N to tune (both precisions)")
     ap.add_argument("--no-small", action="store_true", help="skip the N=200 map (eager vs HIP graph)")
+    ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run this many device copies of 1 GiB (calibrates the HBM byte counters)")
@@ -278,7 +303,20 @@ def main():
     med = G.Medium(3, n, G.HARDSPHERE, (0.05,)).fill(1)
     ks = kgrid(args.grid, args.grid, 5.0, 30.0, -2.0, 0.0)
     # set-up, untimed: the autotuner picks launch shapes for this N in both precisions
-    tuned = None if args.no_tune else {"fp64": G.autotune(n, G.FP64, 512), "fp32": G.autotune(n, G.FP32, 512)}
+    # launch shapes: the tuning table loaded at initialisation (the one shipped beside the
+    # library, or MSGPU_TUNE_FILE), or with --tune the autotuner run here, untimed, saved
+    # to --tune-file when given
+    if args.tune:
+        tuned = {"fp64": G.autotune(n, G.FP64, 512), "fp32": G.autotune(n, G.FP32, 512)}
+        for m in sorted(set(int(x) for x in args.tune_sizes.split(",") if x) - {n}):
+            for mode in (G.FP64, G.FP32):
+                G.autotune(m, mode, 512 if m > 256 else 1024)
+            progress("tuned N=%d" % m)
+        if args.tune_file:
+            G.tune_save(args.tune_file)
+    else:
+        tuned = {"table": G.tuning_table_path()}
+    progress("launch shapes", tuned)
 
     def timed_path(mode):
         for _ in range(args.warmup):
@@ -291,29 +329,46 @@ def main():
         return out, flags, dt, G.profile_end()
 
     out64, _, t64, prof64 = timed_path(G.FP64)
+    progress("fp64 map: %.1f evals/s" % (args.steps * nk / t64))
     fp64_batched = args.steps * nk / t64
-    out_mixed, flags, t_mixed, prof32 = timed_path(G.MIXED)
-    mixed_rate = args.steps * nk / t_mixed
+    if args.no_mixed:   # (profiler runs of the FP64 path alone)
+        out_mixed, flags, t_mixed, prof32 = out64, np.zeros(nk, np.int32), t64, None
+        mixed_rate = None
+    else:
+        out_mixed, flags, t_mixed, prof32 = timed_path(G.MIXED)
+        mixed_rate = args.steps * nk / t_mixed
+        progress("mixed map: %.1f evals/s" % mixed_rate)
     # plain FP64 GPU path: one matrix per call
-    sub = ks[:: max(1, nk // args.plain)][: args.plain]
-    G.tracelog_batch(med, sub[:1], G.FP64)
-    t0 = time.perf_counter()
-    for k in sub:
-        G.tracelog_batch(med, np.array([k]), G.FP64)
-    fp64_plain = len(sub) / (time.perf_counter() - t0)
+    fp64_plain = None
+    if args.plain > 0:
+        sub = ks[:: max(1, nk // args.plain)][: args.plain]
+        G.tracelog_batch(med, sub[:1], G.FP64)
+        t0 = time.perf_counter()
+        for k in sub:
+            G.tracelog_batch(med, np.array([k]), G.FP64)
+        fp64_plain = len(sub) / (time.perf_counter() - t0)
     # arg det = N Im(tracelog) is defined modulo 2 pi: compare Im wrapped
     dre = np.abs(out_mixed.real - out64.real)
     dim = np.abs((n * (out_mixed.imag - out64.imag) + np.pi) % (2 * np.pi) - np.pi) / n
     dtl = np.hypot(dre, dim)
     dtl_re = float(dre.max())
 
-    serving = serving_stream(G, args.requests, args.rate, 100, args.serve_nmax) if args.requests > 0 else None
-    uneven = uneven_search(G, args.uneven, args.uneven_starts, 200, 2000) if args.uneven > 0 else None
-    wave = None if args.no_wave else wavefunction_case(G)
-    small = None if args.no_small else small_map(G, 3)
+    serving = uneven = wave = small = None
+    if args.requests > 0:
+        serving = serving_stream(G, args.requests, args.rate, 100, args.serve_nmax)
+        progress("serving stream done")
+    if args.uneven > 0:
+        uneven = uneven_search(G, args.uneven, args.uneven_starts, 200, 2000)
+        progress("uneven resonance search done")
+    if not args.no_wave:
+        wave = wavefunction_case(G)
+        progress("solve + psi grid done")
+    if not args.no_small:
+        small = small_map(G, 3, args.tune)
+        progress("N=200 map done")
 
     mixed_ok = float(dtl.max()) <= 1e-8
-    if mixed_ok and mixed_rate > fp64_batched:
+    if mixed_rate is not None and mixed_ok and mixed_rate > fp64_batched:
         value, value_path, prof, peak, kname = mixed_rate, "mixed", prof32, PEAK_FP32_MFMA, "gemm_sub_kernel<float>"
         ms_step = 1e3 * t_mixed / args.steps
     else:
@@ -325,13 +380,16 @@ def main():
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": measured_traffic(kname),
                 "kernel": kname + " (LU trailing update, MFMA 16x16x4)",
-                "launches": g["launches"], "avg_ms": g["ms"] / max(1, g["launches"])}
-    g32 = prof32["gemm"]
-    roofline_c64 = {"achieved": g32["flops"] / max(1e-12, g32["ms"] * 1e-3) / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
-                    "frac": g32["flops"] / max(1e-12, g32["ms"] * 1e-3) / PEAK_FP32_MFMA}
+                "launches": g["launches"], "avg_ms": g["ms"] / max(1, g["launches"]),
+                "algorithmic_bytes": g["bytes"] / max(1, g["launches"])}
+    roofline_c64 = None
+    if prof32 is not None:
+        g32 = prof32["gemm"]
+        roofline_c64 = {"achieved": g32["flops"] / max(1e-12, g32["ms"] * 1e-3) / 1e12,
+                        "peak": PEAK_FP32_MFMA / 1e12, "frac": g32["flops"] / max(1e-12, g32["ms"] * 1e-3) / PEAK_FP32_MFMA}
     breakdown = {c: round(v["ms"] / max(1e-9, sum(x["ms"] for x in prof.values())), 4) for c, v in prof.items()}
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(n, 16, 8)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(n, 48, 48)
     line = {
         "metric": "log-det evals/s at N=1000 (mixed vs FP64, same GPU); p50/p99 resonance latency",
         "value": value, "unit": "log-det evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -342,7 +400,7 @@ def main():
                    "cells_per_step": nk},
         "value_path": value_path,
         "fp64_batched_evals_per_s": fp64_batched, "mixed_evals_per_s": mixed_rate,
-        "fp64_plain_evals_per_s": fp64_plain, "value_over_fp64_plain": value / fp64_plain,
+        "fp64_plain_evals_per_s": fp64_plain, "value_over_fp64_plain": value / fp64_plain if fp64_plain else None,
         "mixed_reevaluated_cells": int(flags.sum()), "mixed_reeval_rel": 1e-6,
         "mixed_max_abs_dtracelog": float(dtl.max()), "mixed_median_abs_dtracelog": float(np.median(dtl)),
         "mixed_max_abs_dtracelog_re": dtl_re, "roofline_c64_gemm": roofline_c64,

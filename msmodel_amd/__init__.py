@@ -362,6 +362,16 @@ def autotune(n, mode=FP64, nitems=512):
             "row_major": int(rep[4]), "panel_variant": int(rep[5]), "graph": int(rep[6]), "block128": int(rep[7])}
 
 
+def tuning_table_path():
+    """The tuning table loaded at initialisation: MSGPU_TUNE_FILE, else the table shipped
+    beside the library (None if neither is there)."""
+    env = os.environ.get("MSGPU_TUNE_FILE")
+    if env is not None:
+        return env or None
+    path = os.path.join(os.path.dirname(LIB_PATH), "tuning_gfx950.txt")
+    return path if os.path.exists(path) else None
+
+
 def tune_save(path):
     return _check(lib.msgpu_tune_save(path.encode()), "msgpu_tune_save")
 

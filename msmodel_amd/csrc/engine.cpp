@@ -1,6 +1,7 @@
 // engine.cpp -- host driver of the device path: device buffer pool, stream
 // leases for concurrent callers, and the batched assembly / LU / solve / psi
 // pipelines built from the kernels of kern_*.hip.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -142,6 +143,25 @@ struct Lease {
 static std::once_flag g_once;
 static int g_init_rc = -1;
 
+// The tuning table shipped beside the library (tuning_gfx950.txt, written by
+// msgpu_autotune + msgpu_tune_save on an MI355X) is loaded at initialisation unless
+// MSGPU_TUNE_FILE names another one; MSGPU_TUNE_FILE set to an empty string loads none.
+static void load_initial_tuning() {
+    if (const char* env = getenv("MSGPU_TUNE_FILE")) {
+        if (*env) (void)load_tuning(env);
+        return;
+    }
+    Dl_info info;
+    if (!dladdr((void*)&load_initial_tuning, &info) || !info.dli_fname) return;
+    std::string path(info.dli_fname);
+    const size_t slash = path.rfind('/');
+    path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning_gfx950.txt";
+    if (FILE* f = fopen(path.c_str(), "r")) {
+        fclose(f);
+        (void)load_tuning(path.c_str());
+    }
+}
+
 int engine_init() {
     std::call_once(g_once, [] {
         int n = 0;
@@ -154,7 +174,7 @@ int engine_init() {
             return;
         }
         g_init_rc = 0;
-        if (const char* f = getenv("MSGPU_TUNE_FILE")) (void)load_tuning(f);
+        load_initial_tuning();
     });
     if (g_init_rc != 0) set_error("msgpu: no usable HIP device (hipGetDeviceCount/hipSetDevice failed)");
     return g_init_rc;
