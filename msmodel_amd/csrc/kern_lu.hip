@@ -2,14 +2,13 @@
 // complex matrices (complex128 or complex64), and the kernels that consume it.
 //
 // Per block step k0 (panel width nb):
-//   lu_panel    one workgroup per matrix factors the Np x nb panel.  Rows are
-//               spread over 256 threads (row g belongs to thread (g-c0)%256) and
-//               held in registers, S columns at a time; pivot search is a wave
-//               shuffle argmax followed by a 4-way LDS vote (one barrier per
-//               column).  Pivot rule = LAPACK izamax: largest |Re|+|Im|, first
-//               index on ties.  The panel's interchanges are also folded into a
-//               composite row map (<= 2nb rows) that the next kernel applies as
-//               one gather.
+//   lu_panel    one workgroup per matrix factors the Np x nb panel, S columns
+//               at a time held in registers (row g in thread (g-c0) % PT).  Pivot
+//               search: a DPP arg-max in each wave, then across the waves' keys
+//               after one barrier per column.  Pivot rule = LAPACK izamax: largest
+//               |Re|+|Im|, first index on ties.  The panel's interchanges are also
+//               folded into a composite row map (<= 2nb rows) that the next kernel
+//               applies as one gather.
 //   swap_trsm   one workgroup per 64-column tile: applies the row map, and for
 //               tiles right of the panel solves U12 = L11^-1 A12 in registers
 //               (4 lanes per column, wave shuffles, no barriers).
@@ -54,29 +53,87 @@ __device__ __forceinline__ size_t at(int r, int c, int np) {
 }
 
 // ------------------------------------------------------------------ panel
-// pmap layout per matrix and panel q: [cnt, dst[2*nb], src[2*nb]] (1 + 4*nb ints).
-// One 1024-thread workgroup per matrix; row g of the current sub-panel (columns
-// [c0, c0+S)) lives in thread (g-c0) % PT, register row (g-c0) / PT.
+// pmap layout per matrix and panel q: [cnt, dst[2*nb], src[2*nb]] (1 + 4*nb ints): the
+// panel's interchanges as one move of row src[e] to row dst[e] (rows that end where
+// they started are left out).
+
+// DPP lane exchange (all rows and banks enabled; the controls used have a source lane
+// in the same row for every lane).
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+__device__ __forceinline__ float dpp_v(float v) { return __int_as_float(dpp_i<CTRL>(__float_as_int(v))); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_v(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL>((int)b), hi = dpp_i<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rdlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double rdlane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Pivot order of LAPACK izamax: the larger |Re|+|Im| first, the smaller row on ties.
+template <class T>
+__device__ __forceinline__ void key_max(T& v, int& i, T ov, int oi) {
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+template <int CTRL, class T>
+__device__ __forceinline__ void key_dpp(T& v, int& i) { key_max(v, i, dpp_v<CTRL>(v), dpp_i<CTRL>(i)); }
+// Largest key of each row of 16 lanes, in every lane of the row: quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror.  Called with all lanes active.
+template <class T>
+__device__ __forceinline__ void key_row16(T& v, int& i) {
+    key_dpp<0xB1>(v, i);
+    key_dpp<0x4E>(v, i);
+    key_dpp<0x141>(v, i);
+    key_dpp<0x140>(v, i);
+}
+// Largest key of the wave (wave-uniform result).
+template <class T>
+__device__ __forceinline__ void key_wave(T& v, int& i) {
+    key_row16(v, i);
+    T bv = rdlane(v, 0);
+    int bi = rdlane(i, 0);
+    key_max(bv, bi, rdlane(v, 16), rdlane(i, 16));
+    key_max(bv, bi, rdlane(v, 32), rdlane(i, 32));
+    key_max(bv, bi, rdlane(v, 48), rdlane(i, 48));
+    v = bv;
+    i = bi;
+}
+
+// One workgroup of PT threads per matrix factors the panel's columns [k0, k0+nb),
+// S columns (a sub-panel) at a time: row g of the sub-panel [c0, c0+S) lives in thread
+// (g-c0) % PT, register row (g-c0) / PT.  Per column: the pivot search is a DPP
+// arg-max in each wave and across the waves' results after one barrier; the pivot row
+// travels through LDS.  Per sub-panel: its interchanges are applied to the panel's
+// other columns, U12 of the columns right of it is solved, and those columns get the
+// rank-S update.  The pivots are recorded and the row maps built from them afterwards,
+// one lane per touched row.
 template <class T, int S, int R, int PT, bool RM>
 __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int np, long long mstride,
                                                       int* __restrict__ ipiv_all, int* __restrict__ info_all,
                                                       int* __restrict__ pmap_all, int k0, int nb) {
     constexpr int PW = PT / 64;
-    extern __shared__ int s_slot[];                 // np ints: row -> slot of the composite map
+    static_assert(PW <= 16 && 2 * S <= 64, "panel shape");
     __shared__ T s_abs[2][PW];
     __shared__ int s_idx[2][PW];
     __shared__ cx<T> s_row[2][PW][S];
     __shared__ cx<T> s_rowj[2][S];
     __shared__ cx<T> s_L[S][S];
     __shared__ cx<T> s_U[S][64];
-    __shared__ int s_dst[128], s_src[128], s_cnt;
+    __shared__ int s_piv[64];
     __shared__ int s_sd[2 * S], s_ss[2 * S], s_scnt;
     // L rows of the current sub-panel for the LDS-broadcast rank-S update: for the
-    // 1024-thread shapes (one workgroup per CU anyway), when they fit beside s_slot (4 B
-    // per row, dynamic) and the arrays above within the 160 KB of LDS.  Smaller shapes
-    // keep their LDS small so that several matrices share a CU.
-    constexpr bool LDSU = PT == 1024 && (size_t)PT * R * (S * sizeof(cx<T>) + 4) <= 139264;
-    __shared__ T sLre[LDSU ? S : 1][LDSU ? PT * R : 1], sLim[LDSU ? S : 1][LDSU ? PT * R : 1];
+    // 1024-thread shapes (one workgroup per CU anyway), when they fit beside the arrays
+    // above within the 160 KB of LDS.  Smaller shapes keep their LDS small so that
+    // several matrices share a CU.
+    constexpr bool LDSU = PT == 1024 && (size_t)PT * R * S * sizeof(cx<T>) <= 139264;
+    __shared__ cx<T> sL[LDSU ? S : 1][LDSU ? PT * R : 1];
 
     const int b = blockIdx.x;
     cx<T>* M = A + (size_t)b * mstride;
@@ -85,10 +142,6 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
     int* pmap = pmap_all + ((size_t)b * nq + q) * (1 + 4 * nb);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int kend = min(k0 + nb, np);
-
-    for (int i = tid; i < np; i += PT) s_slot[i] = -1;
-    if (tid == 0) s_cnt = 0;
-    __syncthreads();
 
     for (int c0 = k0; c0 < kend; c0 += S) {
         cx<T> x[R][S];
@@ -116,12 +169,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
                 }
             }
             const int mine = bidx;
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                T ov = __shfl_xor(best, off, 64);
-                int oi = __shfl_xor(bidx, off, 64);
-                if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
-            }
+            key_wave(best, bidx);
             if (lane == 0) { s_abs[buf][wv] = best; s_idx[buf][wv] = bidx; }
             if (br >= 0 && mine == bidx) {
 #pragma unroll
@@ -135,22 +183,17 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
                 for (int c = 0; c < S; c++) s_rowj[buf][(jj + c) % S] = x[0][c];
             }
             __syncthreads();
-            int bw = 0;
-            for (int w = 1; w < PW; w++)
-                if (s_abs[buf][w] > s_abs[buf][bw] || (s_abs[buf][w] == s_abs[buf][bw] && s_idx[buf][w] < s_idx[buf][bw]))
-                    bw = w;
-            const int p = s_idx[buf][bw];
-            const cx<T>* piv = s_row[buf][bw];
+            T wvv = lane < PW ? s_abs[buf][lane] : T(-1);
+            int wvi = lane < PW ? s_idx[buf][lane] : 0x7fffffff;
+            key_row16(wvv, wvi);                        // the PW <= 16 wave keys sit in lanes 0..15
+            const int p = rdlane(wvi, 0);
+            const cx<T>* piv = s_row[buf][((p - c0) % PT) >> 6];
             const cx<T> pv = piv[jj];
             const bool zero = (pv.x == T(0) && pv.y == T(0));
             if (tid == 0) {
                 ipiv[j] = p + 1;
+                s_piv[j - k0] = p;
                 if (zero && info_all[b] == 0) info_all[b] = j + 1;
-                int sj = s_slot[j];
-                if (sj < 0) { sj = s_cnt++; s_slot[j] = sj; s_dst[sj] = j; s_src[sj] = j; }
-                int sp = s_slot[p];
-                if (sp < 0) { sp = s_cnt++; s_slot[p] = sp; s_dst[sp] = p; s_src[sp] = p; }
-                int t = s_src[sj]; s_src[sj] = s_src[sp]; s_src[sp] = t;
             }
             if (p != j) {
                 const int op = (p - c0) % PT, rp = (p - c0) / PT;
@@ -201,18 +244,39 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
         __syncthreads();
         const int nleft = c0 - k0, nright = kend - c0 - S;
         if (nleft + nright > 0) {
-            // composite map of this sub-panel's S interchanges
-            if (tid == 0) {
-                int cnt = 0;
-                for (int jj = 0; jj < S; jj++) {
-                    const int j = c0 + jj, p = ipiv[j] - 1;
-                    int sj = -1, sp = -1;
-                    for (int e = 0; e < cnt; e++) { if (s_sd[e] == j) sj = e; if (s_sd[e] == p) sp = e; }
-                    if (sj < 0) { sj = cnt++; s_sd[sj] = j; s_ss[sj] = j; }
-                    if (sp < 0) { sp = cnt++; s_sd[sp] = p; s_ss[sp] = p; }
-                    int t = s_ss[sj]; s_ss[sj] = s_ss[sp]; s_ss[sp] = t;
+            // composite map of this sub-panel's S interchanges (rows c0 + k <-> pk): lane
+            // k < S takes row c0 + k, lane S + k row pk if below the sub-panel and not
+            // named by an earlier pivot; each traces its row back through the
+            // interchanges to the row whose content ends there
+            if (wv == 0) {
+                int pk[S];
+#pragma unroll
+                for (int k = 0; k < S; k++) pk[k] = s_piv[c0 - k0 + k];
+                int pos = -1;
+                if (lane < S) {
+                    pos = c0 + lane;
+                } else if (lane < 2 * S) {
+                    const int l2 = lane - S, qv = s_piv[c0 - k0 + l2];
+                    bool dup = qv < c0 + S;
+#pragma unroll
+                    for (int k = 0; k < S; k++) dup = dup || (k < l2 && pk[k] == qv);
+                    if (!dup) pos = qv;
                 }
-                s_scnt = cnt;
+                int src = pos;
+#pragma unroll
+                for (int k = S - 1; k >= 0; k--) {
+                    const int a = c0 + k;
+                    if (src == a) src = pk[k];
+                    else if (src == pk[k]) src = a;
+                }
+                const bool act = pos >= 0 && src != pos;
+                const unsigned long long m = __ballot(act);
+                if (act) {
+                    const int slot = __popcll(m & ((1ull << lane) - 1ull));
+                    s_sd[slot] = pos;
+                    s_ss[slot] = src;
+                }
+                if (lane == 0) s_scnt = __popcll(m);
             }
             __syncthreads();
             // apply it to the panel columns outside the sub-panel: gather all, then scatter
@@ -254,26 +318,57 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             }
             __syncthreads();
             if constexpr (LDSU) {
-                // rank-S update with the sub-panel's L rows broadcast through LDS: every thread
-                // updates whole (row, column) elements in storage order, so the loads of
-                // consecutive elements are independent and coalesced
+                // rank-S update with the sub-panel's L rows broadcast through LDS
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const int gl = tid + PT * r;
 #pragma unroll
-                    for (int m = 0; m < S; m++) { sLre[m][gl] = x[r][m].x; sLim[m][gl] = x[r][m].y; }
+                    for (int m = 0; m < S; m++) sL[m][gl] = x[r][m];
                 }
                 __syncthreads();
                 const int nrows = np - c0 - S;
-                const int tot = nrows * nright;
-#pragma unroll 2
-                for (int e = tid; e < tot; e += PT) {
-                    const int ri = RM ? e / nright : e % nrows, ci = RM ? e % nright : e / nrows;
-                    const size_t at_e = at<RM>(c0 + S + ri, c0 + S + ci, np);
-                    cx<T> a = M[at_e];
+                if constexpr (RM) {
+                    // one column per thread (U column in registers), rows strided by the
+                    // number of row groups, UR rows in flight; consecutive threads take
+                    // consecutive columns of a row: contiguous in row-major storage
+                    constexpr int UR = 8;
+                    const int rg = PT / nright, ci = tid % nright, rq = tid / nright;
+                    if (rq < rg) {
+                        cx<T> u[S];
 #pragma unroll
-                    for (int m = 0; m < S; m++) a = a - cx<T>{sLre[m][S + ri], sLim[m][S + ri]} * s_U[m][ci];
-                    M[at_e] = a;
+                        for (int m = 0; m < S; m++) u[m] = s_U[m][ci];
+                        for (int rb = rq; rb < nrows; rb += rg * UR) {
+                            cx<T> a[UR];
+#pragma unroll
+                            for (int t = 0; t < UR; t++) {
+                                const int ri = rb + t * rg;
+                                a[t] = cx<T>{T(0), T(0)};
+                                if (ri < nrows) a[t] = M[at<true>(c0 + S + ri, c0 + S + ci, np)];
+                            }
+#pragma unroll
+                            for (int t = 0; t < UR; t++) {
+                                const int ri = rb + t * rg;
+                                if (ri < nrows) {
+                                    cx<T> v = a[t];
+#pragma unroll
+                                    for (int m = 0; m < S; m++) v = v - sL[m][S + ri] * u[m];
+                                    M[at<true>(c0 + S + ri, c0 + S + ci, np)] = v;
+                                }
+                            }
+                        }
+                    }
+                } else {
+                    // every thread updates whole (row, column) elements in storage order
+                    const int tot = nrows * nright;
+#pragma unroll 2
+                    for (int e = tid; e < tot; e += PT) {
+                        const int ri = e % nrows, ci = e / nrows;
+                        const size_t at_e = at<false>(c0 + S + ri, c0 + S + ci, np);
+                        cx<T> a = M[at_e];
+#pragma unroll
+                        for (int m = 0; m < S; m++) a = a - sL[m][S + ri] * s_U[m][ci];
+                        M[at_e] = a;
+                    }
                 }
                 __syncthreads();
                 continue;
@@ -305,13 +400,41 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             __syncthreads();
         }
     }
-    // publish the panel's composite row map
-    __syncthreads();
-    const int cnt = s_cnt;
-    if (tid == 0) pmap[0] = cnt;
-    for (int e = tid; e < cnt; e += PT) {
-        pmap[1 + e] = s_dst[e];
-        pmap[1 + 2 * nb + e] = s_src[e];
+    // publish the panel's composite row map (wave 0; the last barrier of the loop makes
+    // every pivot visible): lane k takes row k0 + k and row pk when pk >= kend and no
+    // earlier pivot named it, and traces both back through the npv interchanges
+    if (wv == 0) {
+        const int npv = kend - k0;
+        int pa = lane < npv ? k0 + lane : -1, pb = -1;
+        if (lane < npv) {
+            const int qv = s_piv[lane];
+            if (qv >= kend) pb = qv;
+        }
+        int sa = pa, sb = pb;
+        bool dup = false;
+#pragma unroll 8
+        for (int k = npv - 1; k >= 0; k--) {
+            const int a = k0 + k, pk = s_piv[k];
+            if (sa == a) sa = pk;
+            else if (sa == pk) sa = a;
+            if (sb == a) sb = pk;
+            else if (sb == pk) sb = a;
+            dup = dup || (k < lane && pk == pb);
+        }
+        const bool acta = pa >= 0 && sa != pa, actb = pb >= 0 && !dup && sb != pb;
+        const unsigned long long ma = __ballot(acta), mb = __ballot(actb);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (acta) {
+            const int slot = __popcll(ma & below);
+            pmap[1 + slot] = pa;
+            pmap[1 + 2 * nb + slot] = sa;
+        }
+        if (actb) {
+            const int slot = __popcll(ma) + __popcll(mb & below);
+            pmap[1 + slot] = pb;
+            pmap[1 + 2 * nb + slot] = sb;
+        }
+        if (lane == 0) pmap[0] = __popcll(ma) + __popcll(mb);
     }
 }
 
@@ -806,11 +929,11 @@ template <class T, int S, int R, int PT>
 static void panel_go(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb,
                      int nbatch, hipStream_t st, bool rm) {
     if (rm)
-        hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, true>), dim3(nbatch), dim3(PT), sizeof(int) * (size_t)np, st,
+        hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, true>), dim3(nbatch), dim3(PT), 0, st,
                            A, np, mstride, ipiv, info, pmap, k0, nb);
     else
-        hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, false>), dim3(nbatch), dim3(PT), sizeof(int) * (size_t)np,
-                           st, A, np, mstride, ipiv, info, pmap, k0, nb);
+        hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, false>), dim3(nbatch), dim3(PT), 0, st, A, np, mstride,
+                           ipiv, info, pmap, k0, nb);
 }
 
 // Panel shapes <S, R, PT> per size range, tried by the autotuner; variant 0 is the
