@@ -52,9 +52,9 @@ def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
             out, _ = G.tracelog_batch(med, ks, mode)
             if ref is None:
                 ref = out
-            if rm == 0:     # the 128-column blocking accumulates in the same order: exact
+            if rm == 0 and b2 == 0:     # launch shapes only: exact
                 assert np.array_equal(out, ref), (v, chunk, streams, b2)
-            else:   # the row-major factorisation rounds differently: parity tolerance
+            else:   # row-major storage and the 128-column blocking (3M updates) round differently
                 tol = 1e-10 if prec == 0 else 1e-4
                 assert np.abs(out.real - ref.real).max() <= tol * max(1, np.abs(ref.real).max())
                 dim = (330 * (out.imag - ref.imag) + np.pi) % (2 * np.pi) - np.pi
