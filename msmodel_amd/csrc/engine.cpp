@@ -440,22 +440,34 @@ static void lu_solve(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, co
 // of a symmetric M, read column-major as S = M^T = U^T L^T P^T: first U^T W = X
 // (non-unit lower), then L^T V = W (unit upper).  The solution of M Y = X is then
 // Y = P V, i.e. Y[perm[i]] = V[i] with perm from launch_build_perm.
+// Blocks of 128 rows (two 64-row triangular blocks and the 64-row update between them),
+// so that the updates of the rest of X have K = 128 (one pass over X per 128 rows).
 template <class T>
 static void lu_solve_rm(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, long long sx, hipStream_t st) {
     const long long ms = (long long)np * np;
-    const int nq = np / NB;
-    for (int q = 0; q < nq; q++) {
-        const int k0 = q * NB;
-        launch_trsm_block<T>(lu, X, np, ncol, k0, 0, ms, sx, nbatch, st, true);
-        if (k0 + NB < np)
-            launch_gemm_sub<T>(X + k0 + NB, lu + (k0 + NB) + (size_t)k0 * np, X + k0, np - k0 - NB, ncol, NB, np, np,
-                               np, sx, ms, sx, nbatch, st);
+    // X[r0:r0+m] -= S[r0:r0+m, c0:c0+k] X[c0:c0+k]
+    auto upd = [&](int r0, int c0, int m, int k) {
+        if (m > 0 && k > 0)
+            launch_gemm_sub<T>(X + r0, lu + r0 + (size_t)c0 * np, X + c0, m, ncol, k, np, np, np, sx, ms, sx, nbatch,
+                               st);
+    };
+    for (int K0 = 0; K0 < np; K0 += 2 * NB) {
+        const int k1 = K0 + NB, ke = std::min(K0 + 2 * NB, np);
+        launch_trsm_block<T>(lu, X, np, ncol, K0, 0, ms, sx, nbatch, st, true);
+        if (k1 < np) {
+            upd(k1, K0, ke - k1, NB);
+            launch_trsm_block<T>(lu, X, np, ncol, k1, 0, ms, sx, nbatch, st, true);
+        }
+        upd(ke, K0, np - ke, ke - K0);
     }
-    for (int q = nq - 1; q >= 0; q--) {
-        const int k0 = q * NB;
-        launch_trsm_block<T>(lu, X, np, ncol, k0, 1, ms, sx, nbatch, st, true);
-        if (k0 > 0)
-            launch_gemm_sub<T>(X, lu + (size_t)k0 * np, X + k0, k0, ncol, NB, np, np, np, sx, ms, sx, nbatch, st);
+    for (int K0 = ((np - 1) / (2 * NB)) * (2 * NB); K0 >= 0; K0 -= 2 * NB) {
+        const int k1 = K0 + NB, ke = std::min(K0 + 2 * NB, np);
+        if (k1 < np) {
+            launch_trsm_block<T>(lu, X, np, ncol, k1, 1, ms, sx, nbatch, st, true);
+            upd(K0, k1, NB, ke - k1);
+        }
+        launch_trsm_block<T>(lu, X, np, ncol, K0, 1, ms, sx, nbatch, st, true);
+        upd(0, K0, K0, ke - K0);
     }
     HIPCHK(hipGetLastError());
 }
