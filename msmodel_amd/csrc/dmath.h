@@ -37,6 +37,7 @@ template <class T> MSG_HD cx<T> operator+(cx<T> a, T s) { return {a.x + s, a.y};
 template <class T> MSG_HD cx<T> operator-(cx<T> a, T s) { return {a.x - s, a.y}; }
 template <class T> MSG_HD T abs1(cx<T> a) { return fabs(a.x) + fabs(a.y); }
 template <class T> MSG_HD T cabs(cx<T> a) { return hypot(a.x, a.y); }
+template <class T> MSG_HD T norm2(cx<T> a) { return a.x * a.x + a.y * a.y; }
 template <class T> MSG_HD cx<T> conj(cx<T> a) { return {a.x, -a.y}; }
 // Smith's division, robust against overflow of |b|^2.
 template <class T> MSG_HD cx<T> operator/(cx<T> a, cx<T> b) {
@@ -90,17 +91,18 @@ template <class T> MSG_HD void asym_b01(cx<T> z, cx<T>& b0, cx<T>& b1) {
     int kmax = (int)floor(T(2) * cabs(z));
     if (kmax > 80) kmax = 80;
     bool live0 = true, live1 = true;
+    const T e2 = eps_of<T>::v * eps_of<T>::v;      // term tests on squared moduli
     for (int k = 1; k <= kmax && (live0 || live1); k++) {
-        T odd = T(2 * k - 1), q = odd * odd;
+        T odd = T(2 * k - 1), q = odd * odd, rk = T(1) / T(k);
         if (live0) {
-            cx<T> n0 = t0 * iz8 * (-q / T(k));
-            if (cabs(n0) > cabs(t0)) live0 = false;
-            else { t0 = n0; b0 = b0 + t0; if (cabs(t0) < eps_of<T>::v * cabs(b0)) live0 = false; }
+            cx<T> n0 = t0 * iz8 * (-q * rk);
+            if (norm2(n0) > norm2(t0)) live0 = false;
+            else { t0 = n0; b0 = b0 + t0; if (norm2(t0) < e2 * norm2(b0)) live0 = false; }
         }
         if (live1) {
-            cx<T> n1 = t1 * iz8 * ((T(4) - q) / T(k));
-            if (cabs(n1) > cabs(t1)) live1 = false;
-            else { t1 = n1; b1 = b1 + t1; if (cabs(t1) < eps_of<T>::v * cabs(b1)) live1 = false; }
+            cx<T> n1 = t1 * iz8 * ((T(4) - q) * rk);
+            if (norm2(n1) > norm2(t1)) live1 = false;
+            else { t1 = n1; b1 = b1 + t1; if (norm2(t1) < e2 * norm2(b1)) live1 = false; }
         }
     }
 }
@@ -113,18 +115,23 @@ template <class T> MSG_HD void series_k01(cx<T> z, cx<T>& k0, cx<T>& k1) {
     cx<T> t1 = t, i1s = t;
     T hk = T(0), pk = T(-kEuler), pk1 = T(1.0 - kEuler);
     cx<T> s1{pk + pk1, T(0)};
+    // one division per term: rk = 1/k carried from the previous term's 1/(k+1); the stop
+    // test |t| < eps (|i0| + |s0|) is taken on squared moduli (never earlier)
+    const T e2 = eps_of<T>::v * eps_of<T>::v;
+    T rk = T(1);
     for (int k = 1; k < 120; k++) {
-        T fk = T(k);
-        t = t * q * (T(1) / (fk * fk));
-        hk += T(1) / fk;
+        const T rk1 = T(1) / T(k + 1);
+        t = t * q * (rk * rk);
+        hk += rk;
         i0 = i0 + t;
         s0 = s0 + hk * t;
-        t1 = t1 * q * (T(1) / (fk * (fk + T(1))));
+        t1 = t1 * q * (rk * rk1);
         i1s = i1s + t1;
-        pk += T(1) / fk;
-        pk1 += T(1) / (fk + T(1));
+        pk += rk;
+        pk1 += rk1;
         s1 = s1 + (pk + pk1) * t1;
-        if (cabs(t) < eps_of<T>::v * (cabs(i0) + cabs(s0)) && cabs(t1) < eps_of<T>::v * (cabs(i1s) + cabs(s1))) break;
+        if (norm2(t) < e2 * (norm2(i0) + norm2(s0)) && norm2(t1) < e2 * (norm2(i1s) + norm2(s1))) break;
+        rk = rk1;
     }
     k0 = s0 - (lz + T(kEuler)) * i0;
     cx<T> i1 = T(0.5) * (z * i1s);
@@ -152,7 +159,7 @@ template <class T> MSG_HD void cf2_k01(cx<T> z, cx<T>& k0, cx<T>& k1) {
         h = h + delh;
         cx<T> dels = q * delh;
         s = s + dels;
-        if (cabs(dels) < eps_of<T>::v * cabs(s)) break;
+        if (norm2(dels) < eps_of<T>::v * eps_of<T>::v * norm2(s)) break;
     }
     h = a1 * h;
     k0 = csqrt(cx<T>{T(kPi), T(0)} / (T(2) * z)) * cexp(-z) / s;
