@@ -356,7 +356,8 @@ static void factor1(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
 //   panel K0; its interchanges and U12 on [k1, ke) (and on [0, K0) with left);
 //   A(k1:, k1:ke) -= L(k1:, K0:k1) U(K0:k1, k1:ke);  panel k1;
 //   its interchanges on [K0, k1) (on [0, k1) with left);
-//   trailing columns [ke, np): interchanges of K0, U12 of K0, interchanges of k1,
+//   trailing columns [ke, np): interchanges of K0 and of k1 (one composed row map;
+//   those of k1 move rows >= k1 only and so commute with U12 of K0), U12 of K0,
 //   A(k1:ke, ke:) -= L(k1:ke, K0:k1) U(K0:k1, ke:), U12 of k1;
 //   A(ke:, ke:) -= L(ke:, K0:ke) U(K0:ke, ke:)   (K = 128).
 // Row-major storage applies each update to the transposed block (see gemm below).
@@ -382,12 +383,12 @@ static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
         timed(0, 8.0 * nbatch * ((double)(np - k0) * NB * NB / 2), es * nbatch * 2.0 * (np - k0) * NB, st,
               [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st, rm, tn.panel); });
     };
-    auto swap = [&](int k0, int lb, int rb, int re, int do_swap, int do_trsm) {
+    auto swap = [&](int k0, int lb, int rb, int re, int do_swap, int do_trsm, int k0b = -1) {
         const double cols = (double)std::max(0, re - rb) + std::max(0, k0 - lb);
         timed(1, do_trsm ? 8.0 * nbatch * (double)NB * NB / 2 * std::max(0, re - rb) : 0.0,
               es * nbatch * 2.0 * NB * cols, st, [&] {
                   launch_swap_range<T>(mats, np, ms, pmap, k0, NB, lb, rb, re, do_swap, do_trsm, nbatch, st, tn.swap,
-                                       rm);
+                                       rm, k0b);
               });
     };
     for (int K0 = 0; K0 < np; K0 += 2 * NB) {
@@ -403,8 +404,7 @@ static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
         panel(k1);
         swap(k1, left ? 0 : K0, 0, 0, 1, 0);
         if (ke < np) {
-            swap(K0, K0, ke, np, 1, 1);
-            swap(k1, k1, ke, np, 1, 0);
+            swap(K0, K0, ke, np, 1, 1, k1);       // interchanges of K0 and k1 composed, U12 of K0
             gemm(k1, ke, K0, NB, np - ke, NB);
             swap(k1, k1, ke, np, 0, 1);
             gemm(ke, ke, K0, np - ke, np - ke, 2 * NB);

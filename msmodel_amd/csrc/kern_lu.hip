@@ -501,7 +501,8 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
                                                         SwapRange sr) {
     const int nleft_t = sr.nleft_t;
     constexpr int U = (128 * RW + 255) / 256;
-    __shared__ int s_dst[128], s_src[128];
+    __shared__ int s_dst[256], s_src[256];
+    __shared__ int s_d1[128], s_s1[128], s_d2[128], s_s2[128], s_wc[4];
     __shared__ cx<T> sL[64][65];
     const int b = blockIdx.y;
     cx<T>* M = A + (size_t)b * mstride;
@@ -514,8 +515,48 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
     const int colend = left ? min(col0 + TW, k0) : min(col0 + TW, sr.re);
     const int do_trsm = sr.do_trsm;
     const int ncol = colend - col0;
-    const int cnt = pmap[0];
-    for (int e = tid; e < cnt; e += 256) { s_dst[e] = pmap[1 + e]; s_src[e] = pmap[1 + 2 * nb + e]; }
+    int cnt;
+    if (sr.k0b < 0) {
+        cnt = pmap[0];
+        for (int e = tid; e < cnt; e += 256) { s_dst[e] = pmap[1 + e]; s_src[e] = pmap[1 + 2 * nb + e]; }
+    } else {
+        // compose map 1 (panel k0) then map 2 (panel k0b): the row that ends at y is
+        // w = src1'(src2'(y)) (src' = identity off the map's rows), for y in dst1 U dst2;
+        // one thread per candidate (dst2 entries, then dst1 entries not in dst2)
+        const int* pm2 = pmap_all + ((size_t)b * nq + sr.k0b / nb) * (1 + 4 * nb);
+        const int c1 = pmap[0], c2 = pm2[0];
+        for (int e = tid; e < c1; e += 256) { s_d1[e] = pmap[1 + e]; s_s1[e] = pmap[1 + 2 * nb + e]; }
+        for (int e = tid; e < c2; e += 256) { s_d2[e] = pm2[1 + e]; s_s2[e] = pm2[1 + 2 * nb + e]; }
+        __syncthreads();
+        int y = -1, w = -1;
+        bool act = false;
+        if (tid < c1 + c2) {
+            const bool from2 = tid < c2;
+            y = from2 ? s_d2[tid] : s_d1[tid - c2];
+            int j2 = from2 ? tid : -1;
+            if (!from2)
+                for (int e = 0; e < c2; e++) j2 = (s_d2[e] == y) ? e : j2;
+            if (from2 || j2 < 0) {
+                const int z = j2 >= 0 ? s_s2[j2] : y;
+                int j1 = -1;
+                for (int e = 0; e < c1; e++) j1 = (s_d1[e] == z) ? e : j1;
+                w = j1 >= 0 ? s_s1[j1] : z;
+                act = w != y;
+            }
+        }
+        const unsigned long long m = __ballot(act);
+        const int lane = tid & 63, wv = tid >> 6;
+        if (lane == 0) s_wc[wv] = __popcll(m);
+        __syncthreads();
+        int off = 0;
+        for (int v = 0; v < wv; v++) off += s_wc[v];
+        if (act) {
+            const int slot = off + __popcll(m & ((1ull << lane) - 1ull));
+            s_dst[slot] = y;
+            s_src[slot] = w;
+        }
+        cnt = s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
+    }
     if (!left && do_trsm) {
         // unit-lower L11 (nb x nb) of the panel
         for (int e = tid; e < nb * nb; e += 256) {
@@ -525,8 +566,11 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
     }
     __syncthreads();
     if (cnt > 0 && sr.do_swap) {
-        for (int cb = 0; cb < ncol; cb += RW) {
-            const int w = min(RW, ncol - cb), pairs = cnt * w;
+        // columns per round: all rows of a column are read before any is written, and a
+        // round holds at most 256 U elements (a composed map has up to 256 rows)
+        const int rw = max(1, min(RW, (256 * U) / cnt));
+        for (int cb = 0; cb < ncol; cb += rw) {
+            const int w = min(rw, ncol - cb), pairs = cnt * w;
             cx<T> tmp[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -1086,7 +1130,7 @@ void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int 
 
 template <class T>
 void launch_swap_range(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int lb, int rb, int re,
-                       int do_swap, int do_trsm, int nbatch, hipStream_t st, int variant, bool rm) {
+                       int do_swap, int do_trsm, int nbatch, hipStream_t st, int variant, bool rm, int k0b) {
     if (variant < 0 || variant >= kNumSwapShapes) variant = 0;
     const int tw = kSwapShapes[variant].tw;
     SwapRange sr;
@@ -1096,6 +1140,7 @@ void launch_swap_range(cx<T>* A, int np, long long mstride, const int* pmap, int
     sr.re = re;
     sr.do_swap = do_swap;
     sr.do_trsm = do_trsm;
+    sr.k0b = (do_swap && k0b > k0) ? k0b : -1;
     const int nright_t = re > rb ? (re - rb + tw - 1) / tw : 0;
     if (sr.nleft_t + nright_t == 0) return;
     swap_go<T>(variant, dim3(sr.nleft_t + nright_t, nbatch), st, A, np, mstride, pmap, k0, nb, sr, rm);
@@ -1220,7 +1265,7 @@ void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st) {
     template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t, bool, int);   \
     template void launch_swap_trsm<T>(cx<T>*, int, long long, const int*, int, int, int, int, hipStream_t, int, bool);\
     template void launch_swap_range<T>(cx<T>*, int, long long, const int*, int, int, int, int, int, int, int, int,    \
-                                       hipStream_t, int, bool);                                                        \
+                                       hipStream_t, int, bool, int);                                                   \
     template void launch_gemm_sub<T>(cx<T>*, const cx<T>*, const cx<T>*, int, int, int, int, int, int, long long,      \
                                      long long, long long, int, hipStream_t);                                          \
     template void launch_tracelog<T>(const cx<T>*, int, const MatItem*, int, long long, const int*, cx<double>*,     \

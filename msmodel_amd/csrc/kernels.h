@@ -49,12 +49,16 @@ void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int 
 int swap_variants();
 // Row map of panel k0 on explicit column ranges: left tiles cover [lb, k0) (swap only),
 // right tiles [rb, re) (swap if do_swap, then U12 = L11^-1 A12 if do_trsm).
+// k0b >= 0: apply the interchanges of panel k0 and then those of panel k0b (> k0) as one
+// composed row map (panel k0b's rows are all >= k0b, so its interchanges commute with
+// the TRSM of rows [k0, k0 + nb))
 struct SwapRange {
-    int lb, nleft_t, rb, re, do_swap, do_trsm;
+    int lb, nleft_t, rb, re, do_swap, do_trsm, k0b;
 };
 template <class T>
 void launch_swap_range(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int lb, int rb, int re,
-                       int do_swap, int do_trsm, int nbatch, hipStream_t st, int variant = 0, bool rm = false);
+                       int do_swap, int do_trsm, int nbatch, hipStream_t st, int variant = 0, bool rm = false,
+                       int k0b = -1);
 template <class T>
 void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int k, int lda, int ldb, int ldc,
                      long long sC, long long sA, long long sB, int nbatch, hipStream_t st);
