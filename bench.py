@@ -307,10 +307,12 @@ def main():
     # library, or MSGPU_TUNE_FILE), or with --tune the autotuner run here, untimed, saved
     # to --tune-file when given
     if args.tune:
-        tuned = {"fp64": G.autotune(n, G.FP64, 512), "fp32": G.autotune(n, G.FP32, 512)}
+        # (batches of 2048 matrices up to N = 1024: enough chunks for the stream count to show)
+        items = lambda m: 2048 if m <= 1024 else 512
+        tuned = {"fp64": G.autotune(n, G.FP64, items(n)), "fp32": G.autotune(n, G.FP32, items(n))}
         for m in sorted(set(int(x) for x in args.tune_sizes.split(",") if x) - {n}):
             for mode in (G.FP64, G.FP32):
-                G.autotune(m, mode, 512 if m > 256 else 1024)
+                G.autotune(m, mode, items(m))
             progress("tuned N=%d" % m)
         if args.tune_file:
             G.tune_save(args.tune_file)
