@@ -48,16 +48,20 @@ def kgrid(nx, ny, xmin, xmax, ymin, ymax):
     return (xmin + (l % nx) * dx) + 1j * (ymax - (l // nx) * dy)
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch (read + write) of a kernel at this workload, from the counter
-    runs summarised in profiles/pmc_lu_map_n1000.md (FETCH_SIZE and WRITE_SIZE scaled by
-    a 1 GiB copy); None when no such run is recorded."""
+def counter_record(kernel):
+    """The counter-run record of a kernel at this workload (profiles/pmc_lu_map_n1000.md):
+    HBM bytes per launch read and written (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB
+    copy) and the average duration of the kernel alone on the GPU; None if not recorded."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
-        rec = json.load(open(path))["kernels"]["msg::" + kernel]
+        return json.load(open(path))["kernels"]["msg::" + kernel]
     except (OSError, KeyError, ValueError):
         return None
-    return rec["read_bytes_per_launch"] + rec["write_bytes_per_launch"]
+
+
+def measured_traffic(kernel):
+    rec = counter_record(kernel)
+    return None if rec is None else rec["read_bytes_per_launch"] + rec["write_bytes_per_launch"]
 
 
 def cpu_baseline(n, nx, ny):
@@ -384,6 +388,13 @@ def main():
                 "kernel": kname + " (LU trailing update, MFMA 16x16x4)",
                 "launches": g["launches"], "avg_ms": g["ms"] / max(1, g["launches"]),
                 "algorithmic_bytes": g["bytes"] / max(1, g["launches"])}
+    # the same launches alone on the GPU (counter run): the streams' kernels share the GPU
+    # in the timed region, which stretches every launch's duration
+    rec = counter_record(kname)
+    if rec and g["launches"]:
+        fl = g["flops"] / g["launches"]
+        roofline["avg_ms_alone"] = rec["avg_us_alone"] * 1e-3
+        roofline["frac_alone"] = fl / (rec["avg_us_alone"] * 1e-6) / peak
     roofline_c64 = None
     if prof32 is not None:
         g32 = prof32["gemm"]
