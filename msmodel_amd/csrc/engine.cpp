@@ -561,10 +561,33 @@ static Plan make_plan(const std::vector<MediumView>& meds, int nk, const int* mi
     for (int i = 0; i < nk; i++) P.order[i] = i;
     auto np_of = [&](int i) { return meds[midx ? midx[i] : 0].np; };
     std::stable_sort(P.order.begin(), P.order.end(), [&](int a, int b) { return np_of(a) > np_of(b); });
+    // Groups of fewer than kFill matrices leave most CUs idle in the latency-bound panel
+    // phases: such a group takes in the next smaller sizes down to 3/4 of its own, padded
+    // to its size.  The identity padding changes no result bit: padded rows never win a
+    // pivot, every product with a padded row or column is an exact zero, and the blocks
+    // over the real rows stay the same, provided that
+    //  - the storage order and the 64/128-column blocking (tuning table) are the same for
+    //    both sizes, and
+    //  - for the N-column solves of tr(M^-1 M') in complex128, the smaller size is a
+    //    multiple of 128 (the solves' 128-row blocks then end at its last row; otherwise
+    //    its last 64 rows would join a 128-row block, and complex128 updates of K = 128
+    //    round differently from K = 64 ones).
+    constexpr int kFill = 256;
+    const int prec = elem == 16 ? 0 : 1;
+    auto numerics = [&](int np) { const Tune t = tune_for(np, prec); return std::make_pair(t.rm, t.blk2); };
+    auto may_pad = [&](int np2, int np) {
+        return 4 * np2 >= 3 * np && numerics(np2) == numerics(np) &&
+               (mats_per_item == 1 || prec == 1 || np2 % 128 == 0);
+    };
     for (int g0 = 0; g0 < nk;) {
         const int np = np_of(P.order[g0]);
         int g1 = g0;
         while (g1 < nk && np_of(P.order[g1]) == np) g1++;
+        while (g1 < nk && g1 - g0 < kFill) {
+            const int np2 = np_of(P.order[g1]);
+            if (!may_pad(np2, np)) break;
+            while (g1 < nk && np_of(P.order[g1]) == np2) g1++;
+        }
         const int cnt = g1 - g0;
         const int bsz = chunk_size(cnt, max_batch(np, elem, mats_per_item * ns), np, elem == 16 ? 0 : 1, ns);
         for (int s0 = g0; s0 < g1; s0 += bsz) {

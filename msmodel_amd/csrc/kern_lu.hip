@@ -664,8 +664,9 @@ constexpr int PADA = MSG_GEMM_PADA, PADB = MSG_GEMM_PADB;   // LDS row padding (
 //   M3 (complex128, K >= 128): the 3M (Gauss) product.  With A = Ar + i Ai, B = Br + i Bi,
 //     the real products X = Ar Br, Y = Ai Bi, Z = (Ar + Ai)(Br + Bi) give
 //     AB = (X - Y) + i (Z - X - Y): three real MFMA products per complex product instead
-//     of four.  The accumulators start at X = 0, Y = Re C, Z = Re C - Im C, so that
-//     C - AB = (Y - X) + i (X + Y - Z) at the end.  MFMA row i of a 16 x 16 tile is
+//     of four.  The accumulators start at zero and C - AB = (Re C + (Y - X)) +
+//     i (Im C + (X + Y - Z)) is formed at the end (C read there), so that exact-zero
+//     products (padding) leave C bit for bit.  MFMA row i of a 16 x 16 tile is
 //     complex row arow(i): a lane's four registers are four consecutive rows of one
 //     column (one 64-byte access of C).
 //   otherwise: the real product of the interleaved C (2m x n) with [[Ar,-Ai],[Ai,Ar]]
@@ -723,18 +724,13 @@ __device__ __forceinline__ void gemm_tile(T* lds, cx<T>* C, const cx<T>* A, cons
 #pragma unroll
         for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-            for (int cb = 0; cb < 2; cb++) {
-                const int crow = row0 + wr * 32 + rb * 16 + 4 * kq;
-                const int ccol = col0 + wc * 32 + cb * 16 + (lane & 15);
+            for (int cb = 0; cb < 2; cb++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    cx<T> v{T(0), T(0)};
-                    if (ccol < n && crow + r < m) v = C[(size_t)(crow + r) + (size_t)ccol * ldc];
                     X[rb][cb][r] = T(0);
-                    Y[rb][cb][r] = v.x;
-                    Z[rb][cb][r] = v.x - v.y;
+                    Y[rb][cb][r] = T(0);
+                    Z[rb][cb][r] = T(0);
                 }
-            }
         for (int kc0 = 0; kc0 < kk; kc0 += KCT) {
             __syncthreads();
             stage();
@@ -774,7 +770,9 @@ __device__ __forceinline__ void gemm_tile(T* lds, cx<T>* C, const cx<T>* A, cons
                     for (int r = 0; r < 4; r++)
                         if (crow + r < m) {
                             const T x = X[rb][cb][r], y = Y[rb][cb][r], z = Z[rb][cb][r];
-                            C[(size_t)(crow + r) + (size_t)ccol * ldc] = cx<T>{y - x, x + y - z};
+                            cx<T>* cp = C + (size_t)(crow + r) + (size_t)ccol * ldc;
+                            const cx<T> c = *cp;
+                            *cp = cx<T>{c.x + (y - x), c.y + ((x + y) - z)};
                         }
                 }
             }

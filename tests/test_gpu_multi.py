@@ -94,6 +94,37 @@ def test_dk_multi_equals_single_and_oracle(G, media):
         assert abs(got[i] - want) <= 1e-9 * abs(want), (i, got[i], want)
 
 
+# padded sizes 896, 832, 768, 704 (no tuning-table entries: one storage order and blocking),
+# a few items each: the planner factors the small groups together, padded to the largest
+MERGE_SPECS = [(3, 890, 11), (2, 820, 12), (3, 760, 13), (2, 700, 14)]
+
+
+@pytest.fixture(scope="module")
+def merge_media(G):
+    return [G.Medium(d, n, G.HARDSPHERE, (0.1,)).fill(seed) for d, n, seed in MERGE_SPECS]
+
+
+@pytest.mark.parametrize("mode", ["FP64", "FP32", "MIXED"])
+def test_small_groups_padded_together_equal_single(G, merge_media, mode):
+    """Results of media padded to a larger size in a shared batch equal each medium's own
+    call bit for bit (trace-log, and tr(M^-1 M') where its solves allow the padding)."""
+    md = getattr(G, mode)
+    med_of, ks = items(len(merge_media), 3, 21)
+    got, flags = G.tracelog_multi(merge_media, med_of, ks, md, 1e-3)
+    for j, m in enumerate(merge_media):
+        sel = med_of == j
+        want, wflags = G.tracelog_batch(m, ks[sel], md, 1e-3)
+        assert np.array_equal(got[sel], want), (j, np.abs(got[sel] - want).max())
+        assert np.array_equal(flags[sel], wflags)
+    if mode == "MIXED":
+        return
+    dk = G.dk_log_det_multi(merge_media, med_of, ks, md)
+    for j, m in enumerate(merge_media):
+        sel = med_of == j
+        want = G.dk_log_det_batch(m, ks[sel], md)
+        assert np.array_equal(dk[sel], want), (j, np.abs(dk[sel] - want).max())
+
+
 def test_multi_validates_indices(G, media):
     with pytest.raises(G.MsgpuError):
         G.tracelog_multi([media[0][0]], np.array([0, 1], np.int32), np.array([3.0 + 0j, 4.0 + 0j]))
