@@ -1094,20 +1094,16 @@ void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int
         else panel_go<T, 1, 32, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
         return;
     }
+    // up to N = 2048 the defaults are the 1024-thread shapes whose rank-S update reads the
+    // L rows from LDS (at N = 1024-2048 they take 0.4-0.7 of the 512-thread shapes' time)
     if (np <= 1024) {
-        if (variant == 1) {
-            if (D) panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-            else panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        } else if (variant == 2) {
-            panel_go<T, 4, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        } else {
-            if (D) panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-            else panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        }
+        if (variant == 1) panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else if (variant == 2) panel_go<T, 4, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
     } else if (np <= 2048) {
-        if (variant == 1) panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else if (variant == 2) panel_go<T, 4, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else panel_go<T, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (variant == 2) panel_go<T, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else if ((variant == 0) == D) panel_go<T, 4, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        else panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
     } else if (np <= 4096) {     // complex128 default: the 512-thread shape, which does not spill
         if (variant == 1) panel_go<T, 2, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
         else if ((variant == 0) == D) panel_go<T, 4, 8, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
