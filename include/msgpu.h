@@ -159,6 +159,17 @@ int msgpu_dk_log_det_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks
  * with FP64 iterative refinement until ||M a - phi|| <= tol ||phi|| (tol<=0: 1e-12).
  * Returns the number of refinement steps (>=0) or <0 on error. */
 int msgpu_solve(const msgpu_medium* med, const msgpu_iwave* iwave, msgpu_cplx* a, int mode, double tol);
+/* Eigenstates at resonances (replaces the per-root loop of compute_eigenstates,
+ * ms_core.c:296-313, over inverse_iteration, ms_matrix_util.c:279-314): for each root
+ * k_r, the eigenvector of M(k_r) of smallest eigenvalue by inverse power iteration from
+ * the constant unit vector, at most maxit steps, stopping when |v - v_old| < toler; all
+ * roots are factored and iterated in one batch.  vecs: nroot x N (root-major); mu[r] =
+ * the eigenvalue (1/fac of the last step); conv[r] = 1 if converged.  suspect (may be
+ * NULL) holds the flags of check_eigenstate_list (eigenstate_list_util.c:130-163):
+ * |mu| > 1e-6, or an earlier root within 1e-6 |k_r|.  Returns the number of suspect
+ * states, or < 0 on error. */
+int msgpu_eigenstates(const msgpu_medium* med, int nroot, const msgpu_cplx* kroot, int maxit, double toler,
+                      msgpu_cplx* vecs, msgpu_cplx* mu, int* conv, int* suspect);
 /* psi(r) = phi(r) + sum_i a_i G(r|x_i) on an nx x ny grid of the (x0,x1) plane,
  * row-major from the top-left corner (wave_function_util.c:76-81). */
 int msgpu_wavefunction_grid(const msgpu_medium* med, const msgpu_iwave* iwave, const msgpu_cplx* a, int nx,

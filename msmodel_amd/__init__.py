@@ -86,6 +86,8 @@ _SIG = {
     "msgpu_dk_log_det_batch": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
     "msgpu_tracelog_stats": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     "msgpu_solve": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_double]),
+    "msgpu_eigenstates": (ctypes.c_int, [_mp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp, _ip,
+                                         _ip]),
     "msgpu_wavefunction_grid": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                _dp]),
@@ -241,6 +243,19 @@ def dk_log_det_batch(med, ks, mode=FP64):
     out = np.zeros_like(ks)
     _check(lib.msgpu_dk_log_det_batch(med.ref, ks.size, _cp(ks), _cp(out), mode), "msgpu_dk_log_det_batch")
     return out
+
+
+def eigenstates(med, kroot, maxit=50, toler=1e-10):
+    """Eigenvectors of M(k_r) at resonances k_r by inverse power iteration (one batch).
+    Returns (vecs[nroot, N], mu, conv, suspect) with the check_eigenstate_list flags."""
+    kr = np.ascontiguousarray(kroot, dtype=np.complex128)
+    vecs = np.zeros((kr.size, med.n), dtype=np.complex128)
+    mu = np.zeros(kr.size, dtype=np.complex128)
+    conv = np.zeros(kr.size, dtype=np.int32)
+    suspect = np.zeros(kr.size, dtype=np.int32)
+    _check(lib.msgpu_eigenstates(med.ref, kr.size, _cp(kr), maxit, toler, _cp(vecs), _cp(mu),
+                                 conv.ctypes.data_as(_ip), suspect.ctypes.data_as(_ip)), "msgpu_eigenstates")
+    return vecs, mu, conv.astype(bool), suspect.astype(bool)
 
 
 def solve(med, k, source=PLANE, mode=FP64, tol=0.0):

@@ -212,6 +212,28 @@ MSGPU_API int msgpu_wavefunction_grid(const msgpu_medium* med, const msgpu_iwave
                      as_cx(psi));
 }
 
+// Eigenstates at resonances (compute_eigenstates, ms_core.c:296-313) and the checks of
+// check_eigenstate_list (eigenstate_list_util.c:130-163): a state is suspect if
+// |mu| > 1e-6 or an earlier root lies within 1e-6 |k_r| of it.
+MSGPU_API int msgpu_eigenstates(const msgpu_medium* med, int nroot, const msgpu_cplx* kroot, int maxit, double toler,
+                                msgpu_cplx* vecs, msgpu_cplx* mu, int* conv, int* suspect) {
+    if (nroot < 0 || (nroot > 0 && (!kroot || !vecs || !mu || !conv))) {
+        set_error("msgpu_eigenstates: bad arguments");
+        return -2;
+    }
+    if (int rc = eigenstates(view_of(med), nroot, as_cx(kroot), maxit, toler, as_cx(vecs), as_cx(mu), conv)) return rc;
+    constexpr double tolmu = 1e-6;
+    int nsuspect = 0;
+    for (int i = 0; i < nroot; i++) {
+        bool s = std::abs(mu[i]) > tolmu;
+        for (int j = 0; j < i; j++)
+            if (std::abs(kroot[i] - kroot[j]) < tolmu * std::abs(kroot[i])) s = true;
+        if (suspect) suspect[i] = s ? 1 : 0;
+        nsuspect += s ? 1 : 0;
+    }
+    return nsuspect;
+}
+
 // Aberth-Ehrlich on det M(k) of one medium (root_finder.c:522-577); see resonances_any.
 MSGPU_API int msgpu_resonances(const msgpu_medium* med, double xmin, double xmax, double ymin, double ymax, int* nz,
                                msgpu_cplx* z, int nexpsup, int maxit, double toler, int mode) {

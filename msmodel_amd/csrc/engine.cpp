@@ -4,6 +4,7 @@
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1037,6 +1038,79 @@ int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>*
         if (hinfo > 0)
             printf("[LAPACK] Element U(%d,%d) is exactly zero, the solution could not be computed.\n", hinfo, hinfo);
         return steps;
+    });
+}
+
+// ------------------------------------------------------------------ eigenstates
+// Eigenvectors at resonances (compute_eigenstates, ms_core.c:296-313, with
+// inverse_iteration, ms_matrix_util.c:279-314): M(k_r) of all roots assembled and
+// factored in one batch (row-major LU with left interchanges: M^-1 v = P V), then the
+// inverse power iteration on every live vector together, from the constant unit vector
+// (constant_unit_cvector): v <- normalize(M^-1 v) until |v - v_old| < toler or maxit
+// steps; mu = 1/fac of the last step.  One host check per step.
+int eigenstates(const MediumView& mv, int nr, const cx<double>* kr, int maxit, double toler, cx<double>* vecs,
+                cx<double>* mu, int* conv) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    if (nr <= 0) return 0;
+    return guarded([&]() -> int {
+        Lease L;
+        const int np = mv.np, n = mv.n;
+        const long long ms = (long long)np * np;
+        const int bmax = std::max(1, std::min(nr, max_batch(np, sizeof(cx<double>), 1)));
+        DBuf<double> dpos((size_t)n * mv.d);
+        DBuf<cx<double>> dk(bmax), ddiag(bmax), mats((size_t)bmax * ms), V((size_t)bmax * np), X((size_t)bmax * np),
+            Y((size_t)bmax * np), dfac(bmax);
+        DBuf<double> ddelta(bmax);
+        DBuf<int> ipiv((size_t)bmax * np), info(bmax), pmap((size_t)bmax * pmap_ints(np, NB)), perm((size_t)bmax * np),
+            done(bmax);
+        HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+        // constant_unit_cvector (entries 1/(sqrt(n) + 0i)), normalised once more on entry
+        // to inverse_iteration (ms_matrix_util.c:286); zero padding
+        cx<double> u = cx<double>{1.0, 0.0} / cx<double>{std::sqrt((double)n), 0.0};
+        {
+            double r2 = 0.0;
+            for (int i = 0; i < n; i++) r2 += u.x * u.x + u.y * u.y;
+            const cx<double> f = u * (std::sqrt(r2) / std::hypot(u.x, u.y));
+            u = u / f;
+        }
+        std::vector<cx<double>> v0((size_t)bmax * np, cx<double>{0.0, 0.0});
+        for (int b = 0; b < bmax; b++)
+            for (int i = 0; i < n; i++) v0[(size_t)b * np + i] = u;
+        std::vector<cx<double>> hfac(bmax);
+        std::vector<int> hdone(bmax), hinfo(bmax);
+        for (int r0 = 0; r0 < nr; r0 += bmax) {
+            const int B = std::min(bmax, nr - r0);
+            HIPCHK(hipMemcpyAsync(dk.p, kr + r0, sizeof(cx<double>) * B, hipMemcpyHostToDevice, L.st));
+            launch_diag(dk.p, B, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
+            launch_assemble<double>(dpos.p, n, mv.d, np, dk.p, ddiag.p, nullptr, 0, mats.p, ms, B, L.st);
+            factor<double>(mats.p, np, B, ipiv.p, info.p, pmap.p, true, true, L.st);
+            launch_build_perm(pmap.p, np, NB, perm.p, B, L.st);
+            HIPCHK(hipMemcpyAsync(V.p, v0.data(), sizeof(cx<double>) * B * np, hipMemcpyHostToDevice, L.st));
+            launch_fill_int(done.p, 0, B, L.st);
+            std::fill(hdone.begin(), hdone.end(), 0);
+            for (int s = 0; s < maxit; s++) {
+                HIPCHK(hipMemcpyAsync(X.p, V.p, sizeof(cx<double>) * B * np, hipMemcpyDeviceToDevice, L.st));
+                lu_solve_rm<double>(mats.p, X.p, np, 1, B, np, L.st);
+                launch_scatter_rows<double>(Y.p, X.p, perm.p, np, 1, np, np, B, L.st);
+                launch_power_step(V.p, Y.p, n, np, toler, done.p, dfac.p, ddelta.p, B, L.st);
+                HIPCHK(hipMemcpyAsync(hdone.data(), done.p, sizeof(int) * B, hipMemcpyDeviceToHost, L.st));
+                L.sync();
+                if (std::all_of(hdone.begin(), hdone.begin() + B, [](int x) { return x != 0; })) break;
+            }
+            HIPCHK(hipMemcpyAsync(hfac.data(), dfac.p, sizeof(cx<double>) * B, hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpyAsync(hinfo.data(), info.p, sizeof(int) * B, hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpy2DAsync(vecs + (size_t)r0 * n, sizeof(cx<double>) * n, V.p, sizeof(cx<double>) * np,
+                                    sizeof(cx<double>) * n, B, hipMemcpyDeviceToHost, L.st));
+            L.sync();
+            for (int b = 0; b < B; b++) {
+                if (hinfo[b] > 0)
+                    printf("[LAPACK] Element U(%d,%d) is exactly zero, the inverse power iteration will fail.\n",
+                           hinfo[b], hinfo[b]);
+                mu[r0 + b] = cx<double>{1.0, 0.0} / hfac[b];
+                conv[r0 + b] = hdone[b];
+            }
+        }
+        return 0;
     });
 }
 

@@ -1182,6 +1182,69 @@ void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int 
         hipLaunchKernelGGL((trsm_block_kernel<T, 0, false>), grid, dim3(256), 0, st, LU, B, np, ncol, j0, slu, sb);
 }
 
+// One inverse power step's normalisation for every live vector of a batch
+// (complex_vector_util.c:47-74): fac = |y| y_i0 / |y_i0| with i0 the first nonzero entry
+// (fac = 1 for y = 0), y /= fac, delta = |y - v|; then v = y, fac and delta stored, and
+// done set once delta < toler.  Vectors already done are left as they are.  One
+// 256-thread workgroup per vector; sums in FP64.
+__global__ __launch_bounds__(256) void power_step_kernel(cx<double>* __restrict__ V, cx<double>* __restrict__ Y,
+                                                         int n, int ld, double toler, int* __restrict__ done,
+                                                         cx<double>* __restrict__ fac_out,
+                                                         double* __restrict__ delta_out) {
+    __shared__ double s_r[256];
+    __shared__ int s_i[256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    if (done[b]) return;
+    cx<double>* v = V + (size_t)b * ld;
+    cx<double>* y = Y + (size_t)b * ld;
+    double r2 = 0.0;
+    int i0 = n;
+    for (int i = tid; i < n; i += 256) {
+        const cx<double> t = y[i];
+        r2 += t.x * t.x + t.y * t.y;
+        if (i < i0 && (t.x != 0.0 || t.y != 0.0)) i0 = i;
+    }
+    s_r[tid] = r2;
+    s_i[tid] = i0;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) { s_r[tid] += s_r[tid + h]; s_i[tid] = min(s_i[tid], s_i[tid + h]); }
+        __syncthreads();
+    }
+    const int first = s_i[0];
+    cx<double> fac{1.0, 0.0};
+    if (first < n) {
+        const cx<double> p = y[first];
+        fac = (p * (sqrt(s_r[0]) / hypot(p.x, p.y)));
+    }
+    __syncthreads();
+    double d2 = 0.0;
+    for (int i = tid; i < n; i += 256) {
+        const cx<double> t = first < n ? y[i] / fac : y[i];
+        const cx<double> e = t - v[i];
+        d2 += e.x * e.x + e.y * e.y;
+        v[i] = t;
+    }
+    s_r[tid] = d2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) s_r[tid] += s_r[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double delta = sqrt(s_r[0]);
+        fac_out[b] = fac;
+        delta_out[b] = delta;
+        if (delta < toler) done[b] = 1;
+    }
+}
+
+void launch_power_step(cx<double>* V, cx<double>* Y, int n, int ld, double toler, int* done, cx<double>* fac,
+                       double* delta, int nbatch, hipStream_t st) {
+    if (nbatch > 0)
+        hipLaunchKernelGGL(power_step_kernel, dim3(nbatch), dim3(256), 0, st, V, Y, n, ld, toler, done, fac, delta);
+}
+
 // x[i] = v for i < n (a kernel, so that it may be captured in a HIP graph).
 __global__ void fill_int_kernel(int* __restrict__ x, int v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
