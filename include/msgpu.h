@@ -159,6 +159,19 @@ int msgpu_dk_log_det_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks
  * with FP64 iterative refinement until ||M a - phi|| <= tol ||phi|| (tol<=0: 1e-12).
  * Returns the number of refinement steps (>=0) or <0 on error. */
 int msgpu_solve(const msgpu_medium* med, const msgpu_iwave* iwave, msgpu_cplx* a, int mode, double tol);
+/* Random functions and cross sections of many (medium, real k) items in one batch, for
+ * the task drivers run_random_function_omp (ms_core.c:476-519) and
+ * run_differential_cross_section_omp (ms_core.c:526-587); the incident wave is the unit
+ * plane wave along 1x.  item i is (meds[med_of[i]], ks[i]) (med_of may be NULL: medium 0).
+ *   MSGPU_CRSEC: out[i] = total cross section -Im(phi^H M^-1 phi)/k (total_cross_section,
+ *                ms_matrix_util.c:344-366)
+ *   MSGPU_DOS:   out[i] = density of states -Im tr(M^-1 M')/(2 pi k) (ms_core.c:503-505)
+ *   MSGPU_DCS:   out[i*nang + t] = (k/2pi)^(d-3)/(16 pi^2) |b_t^H M^-1 phi|^2 with b_t the
+ *                plane wave at theta_deg[t] degrees (build_plane_wave, ms_matrix_util.c:88-106)
+ * Returns 0, or < 0 on error. */
+enum { MSGPU_CRSEC = 0, MSGPU_DOS = 1, MSGPU_DCS = 2 };
+int msgpu_scattering_multi(int nmed, const msgpu_medium* const* meds, int nk, const int* med_of, const double* ks,
+                           int type, int nang, const double* theta_deg, double* out);
 /* Eigenstates at resonances (replaces the per-root loop of compute_eigenstates,
  * ms_core.c:296-313, over inverse_iteration, ms_matrix_util.c:279-314): for each root
  * k_r, the eigenvector of M(k_r) of smallest eigenvalue by inverse power iteration from

@@ -88,6 +88,8 @@ _SIG = {
     "msgpu_solve": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_double]),
     "msgpu_eigenstates": (ctypes.c_int, [_mp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp, _ip,
                                          _ip]),
+    "msgpu_scattering_multi": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_mp), ctypes.c_int, _ip, _dp, ctypes.c_int,
+                                              ctypes.c_int, _dp, _dp]),
     "msgpu_wavefunction_grid": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                _dp]),
@@ -243,6 +245,23 @@ def dk_log_det_batch(med, ks, mode=FP64):
     out = np.zeros_like(ks)
     _check(lib.msgpu_dk_log_det_batch(med.ref, ks.size, _cp(ks), _cp(out), mode), "msgpu_dk_log_det_batch")
     return out
+
+
+CRSEC, DOS, DCS = 0, 1, 2
+
+
+def scattering_multi(meds, med_of, ks, kind=CRSEC, thetas=None):
+    """Total cross section (CRSEC), density of states (DOS) or differential cross section
+    (DCS, per angle in degrees) of items (meds[med_of[i]], real ks[i]) in one batch."""
+    ks = np.ascontiguousarray(ks, dtype=np.float64)
+    mi = np.ascontiguousarray(med_of, dtype=np.int32)
+    th = np.ascontiguousarray(thetas if thetas is not None else np.zeros(0), dtype=np.float64)
+    nang = th.size if kind == DCS else 0
+    out = np.zeros(ks.size * nang if kind == DCS else ks.size)
+    _check(lib.msgpu_scattering_multi(len(meds), _media(meds), ks.size, mi.ctypes.data_as(_ip), ks.ctypes.data_as(_dp),
+                                      kind, nang, th.ctypes.data_as(_dp), out.ctypes.data_as(_dp)),
+           "msgpu_scattering_multi")
+    return out.reshape(ks.size, nang) if kind == DCS else out
 
 
 def eigenstates(med, kroot, maxit=50, toler=1e-10):

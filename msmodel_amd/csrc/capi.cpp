@@ -212,6 +212,68 @@ MSGPU_API int msgpu_wavefunction_grid(const msgpu_medium* med, const msgpu_iwave
                      as_cx(psi));
 }
 
+// Random functions and cross sections of many (medium, real k) items in one batch:
+// run_random_function_omp (ms_core.c:476-519: total cross section, or the density of
+// states -Im tr(M^-1 M')/(2 pi k)) and run_differential_cross_section_omp
+// (ms_core.c:526-587: prefac |b_theta^H a|^2 per angle).
+MSGPU_API int msgpu_scattering_multi(int nmed, const msgpu_medium* const* meds, int nk, const int* med_of,
+                                     const double* ks, int type, int nang, const double* theta_deg, double* out) {
+    if (nk < 0 || (nk > 0 && (!ks || !out)) || (type == MSGPU_DCS && (nang < 0 || (nang > 0 && !theta_deg)))) {
+        set_error("msgpu_scattering_multi: bad arguments");
+        return -2;
+    }
+    if (type != MSGPU_CRSEC && type != MSGPU_DOS && type != MSGPU_DCS) {
+        set_error("msgpu_scattering_multi: unknown type");
+        return -2;
+    }
+    if (nk == 0) return 0;
+    const std::vector<MediumView> mv = views_of(nmed, meds);
+    std::vector<msgpu_cplx> kc(nk);
+    for (int i = 0; i < nk; i++) kc[i] = msgpu_cplx(ks[i], 0.0);
+    if (type == MSGPU_DOS) {
+        std::vector<msgpu_cplx> dk(nk);
+        if (int rc = dk_log_det_multi(mv, nk, med_of, as_cx(kc.data()), as_cx(dk.data()), 0)) return rc;
+        for (int i = 0; i < nk; i++) out[i] = -dk[i].imag() / (2 * ks[i] * kPi);
+        return 0;
+    }
+    size_t tot = 0;
+    for (int i = 0; i < nk; i++) {
+        const int m = med_of ? med_of[i] : 0;
+        if (m < 0 || m >= nmed) { set_error("msgpu: medium index out of range"); return -2; }
+        tot += mv[m].n;
+    }
+    std::vector<msgpu_cplx> a(tot);
+    if (int rc = amplitudes_multi(mv, nk, med_of, as_cx(kc.data()), as_cx(a.data()))) return rc;
+    size_t off = 0;
+    for (int i = 0; i < nk; i++) {
+        const MediumView& m = mv[med_of ? med_of[i] : 0];
+        const double k = ks[i];
+        const msgpu_cplx* ai = a.data() + off;
+        off += m.n;
+        if (type == MSGPU_CRSEC) {      // -Im(phi^H a)/k, phi_j = e^{i k x_j0} (ms_matrix_util.c:344-366)
+            msgpu_cplx dot = 0.0;
+            for (int j = 0; j < m.n; j++) dot += std::conj(std::exp(msgpu_cplx(0.0, k * m.pos[(size_t)j * m.d]))) * ai[j];
+            out[i] = -dot.imag() / k;
+            continue;
+        }
+        // prefac |b^H a|^2, b the plane wave of angle theta (build_plane_wave, ms_matrix_util.c:88-106)
+        const double prefac = std::pow(k / (2 * kPi), m.d - 3) / (16 * kPi * kPi);
+        for (int t = 0; t < nang; t++) {
+            const double th = theta_deg[t] * kPi / 180;
+            double ct = std::cos(th), st = std::sin(th);
+            if (m.d == 1) ct = ct > 0.0 ? 1.0 : -1.0;
+            msgpu_cplx dot = 0.0;
+            for (int j = 0; j < m.n; j++) {
+                const double* x = m.pos + (size_t)j * m.d;
+                const double ph = m.d == 1 ? ct * x[0] : ct * x[0] + st * x[1];
+                dot += std::conj(std::exp(msgpu_cplx(0.0, 1.0) * msgpu_cplx(k, 0.0) * ph)) * ai[j];
+            }
+            out[(size_t)i * nang + t] = prefac * (dot.real() * dot.real() + dot.imag() * dot.imag());
+        }
+    }
+    return 0;
+}
+
 // Eigenstates at resonances (compute_eigenstates, ms_core.c:296-313) and the checks of
 // check_eigenstate_list (eigenstate_list_util.c:130-163): a state is suspect if
 // |mu| > 1e-6 or an earlier root lies within 1e-6 |k_r| of it.

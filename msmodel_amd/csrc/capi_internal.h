@@ -68,6 +68,8 @@ int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<dou
 int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>* a, int mixed, double tol);
 int eigenstates(const MediumView& mv, int nr, const cx<double>* kr, int maxit, double toler, cx<double>* vecs,
                 cx<double>* mu, int* conv);
+int amplitudes_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                     cx<double>* out);
 // psi grid
 int wave_grid(const MediumView& mv, cx<double> k, int source, const cx<double>* a, int nx, int ny, double xmin,
               double xmax, double ymin, double ymax, cx<double>* psi);

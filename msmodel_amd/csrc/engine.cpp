@@ -1041,6 +1041,52 @@ int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>*
     });
 }
 
+// ------------------------------------------------------------------ amplitudes of many items
+// a = M(k)^-1 phi for items (medium, k) with phi the plane wave along 1x: grouped and
+// chunked like the trace-log batches; each chunk assembled, factored row-major with left
+// interchanges, solved (128-row blocks, one right-hand side per matrix) and permuted
+// (M^-1 phi = P V).  Item q's N_q amplitudes go to out + sum_{q' < q} N_q'.
+int amplitudes_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                     cx<double>* out) {
+    if (nk <= 0) return 0;
+    if (int rc = check_media(meds, nk, midx)) return rc;
+    return guarded([&]() -> int {
+        Lease L;
+        const Plan P = make_plan(meds, nk, midx, sizeof(cx<double>), 1, 1);
+        DeviceItems D(meds, nk, midx, ks, P.order, L.st);
+        DBuf<cx<double>> mats(P.mat_elems), X(P.piv_ints), Y(P.piv_ints);
+        DBuf<int> ipiv(P.piv_ints), info(P.max_b), pmap(P.map_ints), perm(P.piv_ints);
+        std::vector<size_t> off(nk + 1, 0);
+        for (int i = 0; i < nk; i++) off[i + 1] = off[i] + meds[midx ? midx[i] : 0].n;
+        std::vector<cx<double>> hy;
+        std::vector<int> hinfo;
+        for (const Plan::Chunk& ch : P.chunks) {
+            const int B = ch.count, np = ch.np;
+            const long long ms = (long long)np * np;
+            launch_assemble_items<double>(D.items.p + ch.start, np, 0, mats.p, ms, B, L.st);
+            factor<double>(mats.p, np, B, ipiv.p, info.p, pmap.p, true, true, L.st);
+            launch_build_perm(pmap.p, np, NB, perm.p, B, L.st);
+            launch_plane_items(D.items.p + ch.start, np, X.p, B, L.st);
+            lu_solve_rm<double>(mats.p, X.p, np, 1, B, np, L.st);
+            launch_scatter_rows<double>(Y.p, X.p, perm.p, np, 1, np, np, B, L.st);
+            hy.resize((size_t)B * np);
+            hinfo.resize(B);
+            HIPCHK(hipMemcpyAsync(hy.data(), Y.p, sizeof(cx<double>) * B * np, hipMemcpyDeviceToHost, L.st));
+            HIPCHK(hipMemcpyAsync(hinfo.data(), info.p, sizeof(int) * B, hipMemcpyDeviceToHost, L.st));
+            L.sync();
+            for (int q = 0; q < B; q++) {
+                const int i = P.order[ch.start + q];
+                const int n = meds[midx ? midx[i] : 0].n;
+                if (hinfo[q] > 0)
+                    printf("[LAPACK] Element U(%d,%d) is exactly zero, the solution could not be computed.\n",
+                           hinfo[q], hinfo[q]);
+                std::copy(hy.begin() + (size_t)q * np, hy.begin() + (size_t)q * np + n, out + off[i]);
+            }
+        }
+        return 0;
+    });
+}
+
 // ------------------------------------------------------------------ eigenstates
 // Eigenvectors at resonances (compute_eigenstates, ms_core.c:296-313, with
 // inverse_iteration, ms_matrix_util.c:279-314): M(k_r) of all roots assembled and

@@ -186,6 +186,25 @@ void launch_incident(const double* pos, int n, int d, int np, cx<double> k, int 
                        phi);
 }
 
+// Per item of a batch: the unit plane wave along the 1x direction, e^{i k x_0}
+// (ms_matrix_util.c:349-352), padding rows 0; item b's vector at phi + b np.
+__global__ void plane_items_kernel(const MatItem* __restrict__ items, int np, cx<double>* __restrict__ phi) {
+    const int b = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const MatItem& it = items[b];
+    cx<double> v{0.0, 0.0};
+    if (i < it.n) {
+        const double x = it.pos[(size_t)i * it.d];
+        v = cexp(cx<double>{-it.k.y * x, it.k.x * x});
+    }
+    phi[(size_t)b * np + i] = v;
+}
+
+void launch_plane_items(const MatItem* items, int np, cx<double>* phi, int nbatch, hipStream_t st) {
+    if (nbatch > 0)
+        hipLaunchKernelGGL(plane_items_kernel, dim3((np + 255) / 256, nbatch), dim3(256), 0, st, items, np, phi);
+}
+
 // ------------------------------------------------------------------ psi grid
 // psi(r) = phi(r) + sum_i a_i G_d(k, |x_i - r|) at r = (xmin + ix dx, ymax - iy dy, 0, ...)
 // (incident_wave_util.c:87-96, wave_function_util.c:76-81).  One thread per

@@ -20,6 +20,8 @@ struct MatItem {
 
 // kern_assemble.hip
 void launch_item_diag(MatItem* items, int cnt, hipStream_t st);
+// phi[b np + i] = e^{i k_b x_i0} for i < n_b, 0 below np (the plane wave along 1x)
+void launch_plane_items(const MatItem* items, int np, cx<double>* phi, int nbatch, hipStream_t st);
 // kind 0: M, kind 1: dM/dk, from items[0..nbatch)
 template <class T>
 void launch_assemble_items(const MatItem* items, int np, int kind, cx<T>* out, long long mstride, int nbatch,
