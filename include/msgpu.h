@@ -159,6 +159,19 @@ int msgpu_dk_log_det_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks
  * with FP64 iterative refinement until ||M a - phi|| <= tol ||phi|| (tol<=0: 1e-12).
  * Returns the number of refinement steps (>=0) or <0 on error. */
 int msgpu_solve(const msgpu_medium* med, const msgpu_iwave* iwave, msgpu_cplx* a, int mode, double tol);
+/* Roots of det M(k) in the open domain (xmin, xmax) x (ymin, ymax) by the damped Muller
+ * method (find_root_muller, root_finder.c:342-407) on f(k) = smallest eigenvalue of M(k)
+ * (MSGPU_ROOT_MINEIGVAL, find_eigenstates_serial_2, ms_core.c:335-367) or f(k) =
+ * 1/tr(M(k)^-1) (MSGPU_ROOT_INVTRACEINV, find_eigenstates_serial_3, ms_core.c:374-401),
+ * all starts advanced together (one batched evaluation of f per round).  On entry
+ * z[0..*nz) holds the starts; on exit the converged, distinct roots (filter_cvector,
+ * uniq_cvector with relative tolerance 1e-14) and *nz their count.  For MINEIGVAL each
+ * start carries its own inverse-iteration vector from one evaluation to the next (the
+ * reference shares one vector across its sequential loop over starts).  Returns the
+ * number of rounds, or < 0 on error. */
+enum { MSGPU_ROOT_MINEIGVAL = 1, MSGPU_ROOT_INVTRACEINV = 2 };
+int msgpu_roots_muller(const msgpu_medium* med, double xmin, double xmax, double ymin, double ymax, int* nz,
+                       msgpu_cplx* z, int maxit, double toler, int method);
 /* Random functions and cross sections of many (medium, real k) items in one batch, for
  * the task drivers run_random_function_omp (ms_core.c:476-519) and
  * run_differential_cross_section_omp (ms_core.c:526-587); the incident wave is the unit

@@ -90,6 +90,8 @@ _SIG = {
                                          _ip]),
     "msgpu_scattering_multi": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_mp), ctypes.c_int, _ip, _dp, ctypes.c_int,
                                               ctypes.c_int, _dp, _dp]),
+    "msgpu_roots_muller": (ctypes.c_int, [_mp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          _ip, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "msgpu_wavefunction_grid": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                _dp]),
@@ -248,6 +250,17 @@ def dk_log_det_batch(med, ks, mode=FP64):
 
 
 CRSEC, DOS, DCS = 0, 1, 2
+ROOT_MINEIGVAL, ROOT_INVTRACEINV = 1, 2
+
+
+def roots_muller(med, starts, dom, maxit=30, toler=1e-12, method=ROOT_INVTRACEINV):
+    """Roots of det M(k) by damped Muller on 1/tr M^-1 or the smallest eigenvalue of M(k),
+    all starts together; returns (roots, rounds)."""
+    z = np.array(starts, dtype=np.complex128)
+    nz = ctypes.c_int(z.size)
+    rounds = _check(lib.msgpu_roots_muller(med.ref, dom[0], dom[1], dom[2], dom[3], ctypes.byref(nz), _cp(z), maxit,
+                                           toler, method), "msgpu_roots_muller")
+    return z[:nz.value].copy(), rounds
 
 
 def scattering_multi(meds, med_of, ks, kind=CRSEC, thetas=None):

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "aberth.h"
+#include "muller.h"
 #include "capi_internal.h"
 
 using namespace msg;
@@ -272,6 +273,99 @@ MSGPU_API int msgpu_scattering_multi(int nmed, const msgpu_medium* const* meds, 
         }
     }
     return 0;
+}
+
+// Roots of det M(k) by the damped Muller method on another function of M(k), all starts
+// together (find_eigenstates_serial_2/_3, ms_core.c:335-401): every round evaluates f
+// at every pending point of every live start in one batch.
+//   MSGPU_ROOT_INVTRACEINV: f(k) = 1/tr(M(k)^-1)
+//   MSGPU_ROOT_MINEIGVAL:   f(k) = smallest eigenvalue of M(k) by inverse iteration (50
+//     steps, 1e-10) from the start's previous eigenvector (the constant unit vector at
+//     first); a converged root must also have |f| < sqrt(toler) there.
+// Then non-converged starts are dropped (filter_cvector) and duplicates within 1e-14
+// relative (uniq_cvector, complex_vector_util.c:130-177).
+MSGPU_API int msgpu_roots_muller(const msgpu_medium* med, double xmin, double xmax, double ymin, double ymax, int* nz,
+                                 msgpu_cplx* z, int maxit, double toler, int method) {
+    if (!med || !nz || *nz < 0 || (*nz > 0 && !z)) { set_error("msgpu_roots_muller: bad arguments"); return -2; }
+    if (method != MSGPU_ROOT_MINEIGVAL && method != MSGPU_ROOT_INVTRACEINV) {
+        set_error("msgpu_roots_muller: unknown method");
+        return -2;
+    }
+    const MediumView mv = view_of(med);
+    const int nst = *nz, n = mv.n;
+    const bool mineig = method == MSGPU_ROOT_MINEIGVAL;
+    std::vector<MullerRun> runs;
+    for (int i = 0; i < nst; i++) runs.emplace_back(z[i], xmin, xmax, ymin, ymax, maxit, toler);
+    // per start, its current eigenvector (mineigval); the constant unit vector first
+    std::vector<msgpu_cplx> vec(mineig ? (size_t)nst * n : 0, msgpu_cplx(1.0 / std::sqrt((double)n), 0.0));
+    // f at points pts (start owner[j]); eigenvectors of the last point of each start kept
+    auto evaluate = [&](const std::vector<msgpu_cplx>& pts, const std::vector<int>& owner,
+                        std::vector<msgpu_cplx>& f) -> int {
+        const int np_ = (int)pts.size();
+        f.assign(np_, 0.0);
+        if (!mineig) {
+            std::vector<msgpu_cplx> tr(np_);
+            if (int rc = trace_inv_multi({mv}, np_, nullptr, as_cx(pts.data()), as_cx(tr.data()), 0)) return rc;
+            for (int j = 0; j < np_; j++) f[j] = 1.0 / tr[j];
+            return 0;
+        }
+        std::vector<msgpu_cplx> vin((size_t)np_ * n), vout((size_t)np_ * n);
+        std::vector<int> conv(np_);
+        for (int j = 0; j < np_; j++)
+            std::copy(vec.begin() + (size_t)owner[j] * n, vec.begin() + (size_t)(owner[j] + 1) * n,
+                      vin.begin() + (size_t)j * n);
+        if (int rc = eigenstates(mv, np_, as_cx(pts.data()), 50, 1e-10, as_cx(vout.data()), as_cx(f.data()),
+                                 conv.data(), as_cx(vin.data())))
+            return rc;
+        for (int j = 0; j < np_; j++)
+            std::copy(vout.begin() + (size_t)j * n, vout.begin() + (size_t)(j + 1) * n,
+                      vec.begin() + (size_t)owner[j] * n);
+        return 0;
+    };
+    int rounds = 0;
+    for (;;) {
+        std::vector<msgpu_cplx> pts;
+        std::vector<int> owner, first(nst, -1), count(nst, 0);
+        for (int i = 0; i < nst; i++) {
+            if (runs[i].finished()) continue;
+            const std::vector<std::complex<double>> p = runs[i].points();
+            first[i] = (int)pts.size();
+            count[i] = (int)p.size();
+            for (const auto& q : p) { pts.push_back(q); owner.push_back(i); }
+        }
+        if (pts.empty()) break;
+        std::vector<msgpu_cplx> f;
+        if (int rc = evaluate(pts, owner, f)) return rc;
+        for (int i = 0; i < nst; i++)
+            if (count[i]) runs[i].take(std::vector<std::complex<double>>(f.begin() + first[i], f.begin() + first[i] + count[i]));
+        rounds++;
+    }
+    std::vector<int> keep(nst);
+    for (int i = 0; i < nst; i++) keep[i] = runs[i].converged() ? 1 : 0;
+    if (mineig) {       // |mineigval(root)| < sqrt(toler) (ms_core.c:355)
+        std::vector<msgpu_cplx> pts;
+        std::vector<int> owner;
+        for (int i = 0; i < nst; i++)
+            if (keep[i]) { pts.push_back(runs[i].root()); owner.push_back(i); }
+        std::vector<msgpu_cplx> f;
+        if (!pts.empty()) {
+            if (int rc = evaluate(pts, owner, f)) return rc;
+            for (size_t j = 0; j < pts.size(); j++)
+                if (!(std::abs(f[j]) < std::sqrt(toler))) keep[owner[j]] = 0;
+        }
+    }
+    std::vector<msgpu_cplx> roots;
+    for (int i = 0; i < nst; i++) {
+        if (!keep[i]) continue;
+        const msgpu_cplx r = runs[i].root();
+        bool uniq = true;
+        for (const msgpu_cplx& o : roots)
+            if (std::abs(r - o) < 1e-14 * std::abs(r) || std::abs(r - o) < 1e-14) { uniq = false; break; }
+        if (uniq) roots.push_back(r);
+    }
+    std::copy(roots.begin(), roots.end(), z);
+    *nz = (int)roots.size();
+    return rounds;
 }
 
 // Eigenstates at resonances (compute_eigenstates, ms_core.c:296-313) and the checks of

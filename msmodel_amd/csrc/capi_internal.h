@@ -67,7 +67,9 @@ int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<dou
 // Amplitudes a = M(k)^-1 phi; refine: complex64 LU + FP64 refinement to tol.  Returns #steps or <0.
 int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>* a, int mixed, double tol);
 int eigenstates(const MediumView& mv, int nr, const cx<double>* kr, int maxit, double toler, cx<double>* vecs,
-                cx<double>* mu, int* conv);
+                cx<double>* mu, int* conv, const cx<double>* v_init = nullptr);
+int trace_inv_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                    cx<double>* out, int precision);
 int amplitudes_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
                      cx<double>* out);
 // psi grid

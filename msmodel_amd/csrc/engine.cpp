@@ -788,7 +788,7 @@ int tracelog_points(const MediumView& mv, int nk, const cx<double>* ks, cx<doubl
 // ------------------------------------------------------------------ batched d/dk log det
 template <class T>
 static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
-                    cx<double>* out) {
+                    cx<double>* out, bool trace_inv = false) {
     Lanes S(streams_for(max_np(meds, nk, midx), prec_of<T>()));
     const Plan P = make_plan(meds, nk, midx, sizeof(cx<T>), 2, S.ns);
     DeviceItems D(meds, nk, midx, ks, P.order, S[0]);
@@ -812,7 +812,8 @@ static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx
         hipStream_t st = S[ln];
         const MatItem* it = D.items.p + ch.start;
         launch_assemble_items<T>(it, np, 0, mats[ln].p, ms, B, st);
-        launch_assemble_items<T>(it, np, 1, rhs[ln].p, ms, B, st);
+        if (trace_inv) HIPCHK(launch_set_identity<T>(rhs[ln].p, np, ms, B, st));    // tr(M^-1)
+        else launch_assemble_items<T>(it, np, 1, rhs[ln].p, ms, B, st);
         // tr(M^-1 M') = tr(P V) = sum_i V[i, perm[i]] with V from lu_solve_rm
         factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, true, true, st);
         lu_solve_rm<T>(mats[ln].p, rhs[ln].p, np, np, B, ms, st);
@@ -833,6 +834,19 @@ int dk_log_det_multi(const std::vector<MediumView>& meds, int nk, const int* mid
     return guarded([&]() -> int {
         if (precision == 1) dk_impl<float>(meds, nk, midx, ks, out);
         else dk_impl<double>(meds, nk, midx, ks, out);
+        return 0;
+    });
+}
+
+// tr(M(k)^-1) (traceinv, ms_matrix_util.c:319-335): the tr(M^-1 M') pipeline with the
+// identity in place of M'.
+int trace_inv_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
+                    cx<double>* out, int precision) {
+    if (nk <= 0) return 0;
+    if (int rc = check_media(meds, nk, midx)) return rc;
+    return guarded([&]() -> int {
+        if (precision == 1) dk_impl<float>(meds, nk, midx, ks, out, true);
+        else dk_impl<double>(meds, nk, midx, ks, out, true);
         return 0;
     });
 }
@@ -1095,7 +1109,7 @@ int amplitudes_multi(const std::vector<MediumView>& meds, int nk, const int* mid
 // (constant_unit_cvector): v <- normalize(M^-1 v) until |v - v_old| < toler or maxit
 // steps; mu = 1/fac of the last step.  One host check per step.
 int eigenstates(const MediumView& mv, int nr, const cx<double>* kr, int maxit, double toler, cx<double>* vecs,
-                cx<double>* mu, int* conv) {
+                cx<double>* mu, int* conv, const cx<double>* v_init) {
     if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
     if (nr <= 0) return 0;
     return guarded([&]() -> int {
@@ -1131,6 +1145,21 @@ int eigenstates(const MediumView& mv, int nr, const cx<double>* kr, int maxit, d
             launch_assemble<double>(dpos.p, n, mv.d, np, dk.p, ddiag.p, nullptr, 0, mats.p, ms, B, L.st);
             factor<double>(mats.p, np, B, ipiv.p, info.p, pmap.p, true, true, L.st);
             launch_build_perm(pmap.p, np, NB, perm.p, B, L.st);
+            if (v_init) {       // warm starts, normalised on entry as inverse_iteration does
+                for (int b = 0; b < B; b++) {
+                    const cx<double>* w = v_init + (size_t)(r0 + b) * n;
+                    cx<double>* t = v0.data() + (size_t)b * np;
+                    double r2 = 0.0;
+                    int i0 = n;
+                    for (int i = 0; i < n; i++) {
+                        r2 += w[i].x * w[i].x + w[i].y * w[i].y;
+                        if (i0 == n && (w[i].x != 0.0 || w[i].y != 0.0)) i0 = i;
+                    }
+                    const cx<double> f = i0 < n ? w[i0] * (std::sqrt(r2) / std::hypot(w[i0].x, w[i0].y))
+                                                : cx<double>{1.0, 0.0};
+                    for (int i = 0; i < n; i++) t[i] = i0 < n ? w[i] / f : w[i];
+                }
+            }
             HIPCHK(hipMemcpyAsync(V.p, v0.data(), sizeof(cx<double>) * B * np, hipMemcpyHostToDevice, L.st));
             launch_fill_int(done.p, 0, B, L.st);
             std::fill(hdone.begin(), hdone.end(), 0);

@@ -1245,6 +1245,24 @@ void launch_power_step(cx<double>* V, cx<double>* Y, int n, int ld, double toler
         hipLaunchKernelGGL(power_step_kernel, dim3(nbatch), dim3(256), 0, st, V, Y, n, ld, toler, done, fac, delta);
 }
 
+// A[b] = I (np x np) for every matrix of a batch: zero fill, then the diagonal.
+template <class T>
+__global__ void unit_diag_kernel(cx<T>* __restrict__ A, int np, long long ms) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) A[(size_t)blockIdx.y * ms + (size_t)i * (np + 1)] = cx<T>{T(1), T(0)};
+}
+
+template <class T>
+hipError_t launch_set_identity(cx<T>* A, int np, long long ms, int nbatch, hipStream_t st) {
+    if (nbatch <= 0) return hipSuccess;
+    const hipError_t e = hipMemsetAsync(A, 0, sizeof(cx<T>) * (size_t)ms * nbatch, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(unit_diag_kernel<T>, dim3((np + 255) / 256, nbatch), dim3(256), 0, st, A, np, ms);
+    return hipGetLastError();
+}
+template hipError_t launch_set_identity<double>(cx<double>*, int, long long, int, hipStream_t);
+template hipError_t launch_set_identity<float>(cx<float>*, int, long long, int, hipStream_t);
+
 // x[i] = v for i < n (a kernel, so that it may be captured in a HIP graph).
 __global__ void fill_int_kernel(int* __restrict__ x, int v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -79,6 +79,9 @@ void launch_trsm_block(const cx<T>* LU, cx<T>* B, int np, int ncol, int j0, int 
                        int nbatch, hipStream_t st, bool rm = false);
 // x[i] = v, i < n
 void launch_fill_int(int* x, int v, int n, hipStream_t st);
+// every matrix of the batch set to the identity (not for captured graphs: a memset)
+template <class T>
+hipError_t launch_set_identity(cx<T>* A, int np, long long ms, int nbatch, hipStream_t st);
 // inverse power iteration: v <- y / fac, fac = |y| y_i0/|y_i0|, delta = |v_new - v_old|,
 // done once delta < toler (vectors of ld entries, n used; done vectors untouched)
 void launch_power_step(cx<double>* V, cx<double>* Y, int n, int ld, double toler, int* done, cx<double>* fac,
