@@ -389,12 +389,14 @@ def main():
                 "launches": g["launches"], "avg_ms": g["ms"] / max(1, g["launches"]),
                 "algorithmic_bytes": g["bytes"] / max(1, g["launches"])}
     # the same launches alone on the GPU (counter run): the streams' kernels share the GPU
-    # in the timed region, which stretches every launch's duration
+    # in the timed region, which stretches every launch's duration; reported when this
+    # run's launches are the recorded ones (same flops per launch)
     rec = counter_record(kname)
-    if rec and g["launches"]:
+    if rec and g["launches"] and "gflop_per_launch" in rec:
         fl = g["flops"] / g["launches"]
-        roofline["avg_ms_alone"] = rec["avg_us_alone"] * 1e-3
-        roofline["frac_alone"] = fl / (rec["avg_us_alone"] * 1e-6) / peak
+        if abs(fl / (rec["gflop_per_launch"] * 1e9) - 1) < 0.01:
+            roofline["avg_ms_alone"] = rec["avg_us_alone"] * 1e-3
+            roofline["frac_alone"] = fl / (rec["avg_us_alone"] * 1e-6) / peak
     roofline_c64 = None
     if prof32 is not None:
         g32 = prof32["gemm"]
