@@ -53,7 +53,8 @@ void launch_item_diag(MatItem* items, int cnt, hipStream_t st) {
 // Green function once per unordered pair and writes both (I,J) and, through an
 // LDS transpose, (J,I), so M = M^T exactly (ms_matrix_util.c:36-41 mirrors the
 // upper triangle the same way).
-constexpr int AT = 32;
+constexpr int AT = 32;                 // tile edge (256 threads: AT / (256 / AT) elements each)
+constexpr int ARW = 256 / AT;           // thread rows of a tile
 
 template <class T>
 __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict__ pos, int n, int d, int np,
@@ -95,8 +96,8 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
     cx<T>* M = out + (size_t)b * mstride;
     const int ii = tid % AT, j0 = tid / AT;
 #pragma unroll
-    for (int q = 0; q < AT / 8; q++) {
-        const int jj = j0 + 8 * q;
+    for (int q = 0; q < AT / ARW; q++) {
+        const int jj = j0 + ARW * q;
         const int gi = I * AT + ii, gj = J * AT + jj;
         cx<T> v;
         if (gi >= n || gj >= n) {
@@ -126,8 +127,8 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
     if (I == J) return;
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < AT / 8; q++) {
-        const int bb = j0 + 8 * q;        // column inside tile (J, I) is I*AT + bb
+    for (int q = 0; q < AT / ARW; q++) {
+        const int bb = j0 + ARW * q;      // column inside tile (J, I) is I*AT + bb
         M[(size_t)(J * AT + ii) + (size_t)(I * AT + bb) * np] = tile[ii][bb];
     }
 }
