@@ -358,8 +358,9 @@ static void factor1(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
 //   A(k1:, k1:ke) -= L(k1:, K0:k1) U(K0:k1, k1:ke);  panel k1;
 //   its interchanges on [K0, k1) (on [0, k1) with left);
 //   trailing columns [ke, np): interchanges of K0 and of k1 (one composed row map;
-//   those of k1 move rows >= k1 only and so commute with U12 of K0), U12 of K0,
-//   A(k1:ke, ke:) -= L(k1:ke, K0:k1) U(K0:k1, ke:), U12 of k1;
+//   those of k1 move rows >= k1 only and so commute with U12 of K0), U12 of K0, then
+//   U12 of k1 = L11(k1)^-1 (A(k1:ke, ke:) - L(k1:ke, K0:k1) U(K0:k1, ke:)) as one update
+//   of K = 128 with the 64 x 128 block Wb = [L11^-1 L21, I - L11^-1] (block_w_kernel);
 //   A(ke:, ke:) -= L(ke:, K0:ke) U(K0:ke, ke:)   (K = 128).
 // Row-major storage applies each update to the transposed block (see gemm below).
 template <class T>
@@ -370,6 +371,8 @@ static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
     launch_fill_int(info, 0, nbatch, st);
     const double es = sizeof(cx<T>);
     const Tune tn = tune_for(np, prec_of<T>());
+    // the update blocks Wb follow the row maps in the factorisation workspace
+    cx<T>* wblk = reinterpret_cast<cx<T>*>(pmap + wblock_offset(np, NB, nbatch));
     // M(r0:r0+m, c0:c0+n) -= M(r0:, p0:p0+k) M(p0:, c0:).  Row-major storage read column-major
     // is S = M^T, where this is S(c0:, r0:) -= S(c0:, p0:) S(p0:, r0:): the transposed block.
     auto gemm = [&](int r0, int c0, int p0, int m, int n, int k) {
@@ -406,8 +409,16 @@ static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
         swap(k1, left ? 0 : K0, 0, 0, 1, 0);
         if (ke < np) {
             swap(K0, K0, ke, np, 1, 1, k1);       // interchanges of K0 and k1 composed, U12 of K0
-            gemm(k1, ke, K0, NB, np - ke, NB);
-            swap(k1, k1, ke, np, 0, 1);
+            // U12 of k1 = L11^-1 (A - L21 U12(K0)) as one K = 128 update with Wb
+            launch_block_w<T>(mats, np, ms, K0, wblk, nbatch, st, rm);
+            timed(2, 8.0 * nbatch * (double)NB * (np - ke) * 2 * NB, es * nbatch * (2.0 * NB * (np - ke) + (NB + np - ke) * 2.0 * NB), st, [&] {
+                if (rm)
+                    launch_gemm_sub<T>(mats + ke + (size_t)k1 * np, mats + ke + (size_t)K0 * np, wblk, np - ke, NB,
+                                       2 * NB, np, 2 * NB, np, ms, ms, 64 * 128, nbatch, st);
+                else
+                    launch_gemm_sub<T>(mats + k1 + (size_t)ke * np, wblk, mats + K0 + (size_t)ke * np, NB, np - ke,
+                                       2 * NB, NB, np, np, ms, 64 * 128, ms, nbatch, st);
+            });
             gemm(ke, ke, K0, np - ke, np - ke, 2 * NB);
         }
     }
@@ -596,7 +607,7 @@ static Plan make_plan(const std::vector<MediumView>& meds, int nk, const int* mi
             P.chunks.push_back({np, s0, B});
             P.mat_elems = std::max(P.mat_elems, (size_t)B * np * np);
             P.piv_ints = std::max(P.piv_ints, (size_t)B * np);
-            P.map_ints = std::max(P.map_ints, (size_t)B * pmap_ints(np, NB));
+            P.map_ints = std::max(P.map_ints, factor_ws_ints(np, NB, B));
             P.max_b = std::max(P.max_b, B);
         }
         g0 = g1;
@@ -677,7 +688,7 @@ static ChunkGraph<T>* chunk_graph(hipStream_t st, int np, int B, bool rm, const 
     cg->mats.alloc((size_t)B * ms);
     cg->ipiv.alloc((size_t)B * np);
     cg->info.alloc(B);
-    cg->pmap.alloc((size_t)B * pmap_ints(np, NB));
+    cg->pmap.alloc(factor_ws_ints(np, NB, B));
     cg->out.alloc(B);
     cg->stats.alloc(2 * (size_t)B);
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1009,7 +1020,7 @@ int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>*
         DBuf<double> dpos((size_t)n * mv.d);
         DBuf<cx<double>> dk(1), ddiag(1), phi(np), x(np), r(np);
         DBuf<cx<double>> m64(ms);
-        DBuf<int> ipiv(np), info(1), pmap(pmap_ints(np, NB)), perm(np);
+        DBuf<int> ipiv(np), info(1), pmap(factor_ws_ints(np, NB, 1)), perm(np);
         HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpyAsync(dk.p, &k, sizeof(k), hipMemcpyHostToDevice, L.st));
         launch_diag(dk.p, 1, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
@@ -1121,7 +1132,7 @@ int eigenstates(const MediumView& mv, int nr, const cx<double>* kr, int maxit, d
         DBuf<cx<double>> dk(bmax), ddiag(bmax), mats((size_t)bmax * ms), V((size_t)bmax * np), X((size_t)bmax * np),
             Y((size_t)bmax * np), dfac(bmax);
         DBuf<double> ddelta(bmax);
-        DBuf<int> ipiv((size_t)bmax * np), info(bmax), pmap((size_t)bmax * pmap_ints(np, NB)), perm((size_t)bmax * np),
+        DBuf<int> ipiv((size_t)bmax * np), info(bmax), pmap(factor_ws_ints(np, NB, bmax)), perm((size_t)bmax * np),
             done(bmax);
         HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
         // constant_unit_cvector (entries 1/(sqrt(n) + 0i)), normalised once more on entry
@@ -1294,7 +1305,7 @@ int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, i
         Lease L;
         const int np = padded(n);
         DBuf<cx<double>> m((size_t)np * np), x((size_t)np * (nrhs > 0 ? nrhs : 1)), tl(1);
-        DBuf<int> dpiv(np), dinfo(1), pmap(pmap_ints(np, NB));
+        DBuf<int> dpiv(np), dinfo(1), pmap(factor_ws_ints(np, NB, 1));
         upload_padded(m.p, n, np, a, lda, L.st);
         if (nrhs > 0) {
             HIPCHK(hipMemsetAsync(x.p, 0, sizeof(cx<double>) * (size_t)np * nrhs, L.st));

@@ -616,6 +616,67 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
     }
 }
 
+// ------------------------------------------------------------------ second-panel update block
+// For a 128-column block [K0, K0+128) with second panel k1 = K0 + 64: the rows k1..ke
+// of the trailing columns become U12(k1) = L11^-1 (A - L21 U12(K0)) with L11 the unit
+// lower block of k1 and L21 = M(k1:ke, K0:k1).  That is one update of K = 128,
+//   C <- C - Wb [U12(K0); C],   Wb = [L11^-1 L21, I - L11^-1]   (64 x 128),
+// on the rows K0..ke of the trailing columns.  One 256-thread workgroup per matrix forms
+// Wb: L11^-1 by forward substitution of the identity (4 lanes per column), then
+// L11^-1 L21.  Wb is stored column-major (ld 64) for column-major factors and row-major
+// (row stride 128) for row-major ones, where the update is applied to the transposed
+// blocks and Wb^T is the right operand.
+template <class T, bool RM>
+__global__ __launch_bounds__(256) void block_w_kernel(const cx<T>* __restrict__ A, int np, long long mstride, int K0,
+                                                      cx<T>* __restrict__ W) {
+    __shared__ cx<T> sL[64][65];
+    __shared__ cx<T> sB[64][65];
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const cx<T>* M = A + (size_t)b * mstride;
+    cx<T>* Wb = W + (size_t)b * (64 * 128);
+    const int k1 = K0 + 64;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int i = RM ? e / 64 : e % 64, m = RM ? e % 64 : e / 64;
+        sL[i][m] = M[at<RM>(k1 + i, k1 + m, np)];
+        sB[i][m] = M[at<RM>(k1 + i, K0 + m, np)];
+    }
+    __syncthreads();
+    const int cl = tid >> 2, qq = tid & 3;
+    cx<T> xv[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) xv[m] = (4 * m + qq == cl) ? cx<T>{T(1), T(0)} : cx<T>{T(0), T(0)};
+    lower_unit_solve<T>(xv, sL, 64, qq, lane);       // column cl of L11^-1
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++) sL[4 * m + qq][cl] = xv[m];
+    __syncthreads();
+    // thread: row i = tid / 4, columns p = tid % 4 + 4 t
+    const int i = tid >> 2;
+#pragma unroll 4
+    for (int t = 0; t < 16; t++) {
+        const int p = (tid & 3) + 4 * t;
+        cx<T> acc{T(0), T(0)};
+        for (int j = 0; j <= i; j++) acc = acc + sL[i][j] * sB[j][p];
+        const cx<T> ip = (p == i) ? cx<T>{T(0), T(0)} : cx<T>{-sL[i][p].x, -sL[i][p].y};   // (I - L11^-1)[i][p]
+        if (RM) {
+            Wb[i * 128 + p] = acc;
+            Wb[i * 128 + 64 + p] = ip;
+        } else {
+            Wb[i + p * 64] = acc;
+            Wb[i + (64 + p) * 64] = ip;
+        }
+    }
+}
+
+template <class T>
+void launch_block_w(const cx<T>* A, int np, long long mstride, int K0, cx<T>* W, int nbatch, hipStream_t st, bool rm) {
+    if (nbatch <= 0) return;
+    if (rm) hipLaunchKernelGGL((block_w_kernel<T, true>), dim3(nbatch), dim3(256), 0, st, A, np, mstride, K0, W);
+    else hipLaunchKernelGGL((block_w_kernel<T, false>), dim3(nbatch), dim3(256), 0, st, A, np, mstride, K0, W);
+}
+template void launch_block_w<double>(const cx<double>*, int, long long, int, cx<double>*, int, hipStream_t, bool);
+template void launch_block_w<float>(const cx<float>*, int, long long, int, cx<float>*, int, hipStream_t, bool);
+
 // Tile width and round width of the swap + TRSM variants (the autotuner picks one).
 struct SwapShape { int tw, rw; };
 constexpr SwapShape kSwapShapes[] = {{256, 8}, {128, 16}, {64, 16}, {64, 32}, {128, 32}, {256, 32}};

@@ -111,5 +111,13 @@ void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st);
 
 // pmap ints per matrix for a given (np, nb)
 inline size_t pmap_ints(int np, int nb) { return (size_t)(np / nb) * (1 + 4 * (size_t)nb); }
+// The factorisation workspace of a batch: the row maps of every matrix, then (16-byte
+// aligned) one 64 x 128 complex128 block per matrix for the 128-column blocking.
+constexpr size_t kBlockWInts = 64 * 128 * 4;
+inline size_t wblock_offset(int np, int nb, int nbatch) { return ((size_t)nbatch * pmap_ints(np, nb) + 63) / 64 * 64; }
+inline size_t factor_ws_ints(int np, int nb, int nbatch) { return wblock_offset(np, nb, nbatch) + (size_t)nbatch * kBlockWInts; }
+// Wb of the second panel of the 128-column block at K0 (see kern_lu.hip)
+template <class T>
+void launch_block_w(const cx<T>* A, int np, long long mstride, int K0, cx<T>* W, int nbatch, hipStream_t st, bool rm);
 
 }  // namespace msg
