@@ -383,9 +383,9 @@ static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
                                rm ? n : m, rm ? m : n, k, np, np, np, ms, ms, ms, nbatch, st);
         });
     };
-    auto panel = [&](int k0) {
-        timed(0, 8.0 * nbatch * ((double)(np - k0) * NB * NB / 2), es * nbatch * 2.0 * (np - k0) * NB, st,
-              [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st, rm, tn.panel); });
+    auto panel = [&](int k0, int xa = 0, int xb = 0) {
+        timed(0, 8.0 * nbatch * ((double)(np - k0) * NB * NB / 2), es * nbatch * 2.0 * (np - k0 + (xb - xa)) * NB, st,
+              [&] { launch_panel<T>(mats, np, ms, ipiv, info, pmap, k0, NB, nbatch, st, rm, tn.panel, xa, xb); });
     };
     auto swap = [&](int k0, int lb, int rb, int re, int do_swap, int do_trsm, int k0b = -1) {
         const double cols = (double)std::max(0, re - rb) + std::max(0, k0 - lb);
@@ -405,8 +405,12 @@ static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
         }
         swap(K0, left ? 0 : K0, k1, ke, 1, 1);
         gemm(k1, k1, K0, np - k1, NB, NB);
-        panel(k1);
-        swap(k1, left ? 0 : K0, 0, 0, 1, 0);
+        if (left) {
+            panel(k1);
+            swap(k1, 0, 0, 0, 1, 0);
+        } else {
+            panel(k1, K0, k1);                    // its interchanges on [K0, k1) in the same launch
+        }
         if (ke < np) {
             swap(K0, K0, ke, np, 1, 1, k1);       // interchanges of K0 and k1 composed, U12 of K0
             // U12 of k1 = L11^-1 (A - L21 U12(K0)) as one K = 128 update with Wb

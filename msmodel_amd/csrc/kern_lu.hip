@@ -117,7 +117,7 @@ __device__ __forceinline__ void key_wave(T& v, int& i) {
 template <class T, int S, int R, int PT, bool RM>
 __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int np, long long mstride,
                                                       int* __restrict__ ipiv_all, int* __restrict__ info_all,
-                                                      int* __restrict__ pmap_all, int k0, int nb) {
+                                                      int* __restrict__ pmap_all, int k0, int nb, int xa, int xb) {
     constexpr int PW = PT / 64;
     static_assert(PW <= 16 && 2 * S <= 64, "panel shape");
     __shared__ T s_abs[2][PW];
@@ -128,6 +128,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
     __shared__ cx<T> s_U[S][64];
     __shared__ int s_piv[64];
     __shared__ int s_sd[2 * S], s_ss[2 * S], s_scnt;
+    __shared__ int s_pd[128], s_ps[128], s_pcnt;        // the panel's composite map, for columns [xa, xb)
     // L rows of the current sub-panel for the LDS-broadcast rank-S update: for the
     // 1024-thread shapes (one workgroup per CU anyway), when they fit beside the arrays
     // above within the 160 KB of LDS.  Smaller shapes keep their LDS small so that
@@ -428,13 +429,51 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             const int slot = __popcll(ma & below);
             pmap[1 + slot] = pa;
             pmap[1 + 2 * nb + slot] = sa;
+            s_pd[slot] = pa;
+            s_ps[slot] = sa;
         }
         if (actb) {
             const int slot = __popcll(ma) + __popcll(mb & below);
             pmap[1 + slot] = pb;
             pmap[1 + 2 * nb + slot] = sb;
+            s_pd[slot] = pb;
+            s_ps[slot] = sb;
         }
-        if (lane == 0) pmap[0] = __popcll(ma) + __popcll(mb);
+        if (lane == 0) {
+            pmap[0] = __popcll(ma) + __popcll(mb);
+            s_pcnt = __popcll(ma) + __popcll(mb);
+        }
+    }
+    if (xb > xa) {
+        // the panel's interchanges on the extra columns [xa, xb) (at most 64, outside the
+        // panel): every row of a round's columns is read before any is written
+        __syncthreads();
+        const int cnt = s_pcnt, ncol = xb - xa;
+        if (cnt == 0) return;
+        const int rw = max(1, min(ncol, (4 * PT) / cnt));
+        for (int cb = 0; cb < ncol; cb += rw) {
+            const int w = min(rw, ncol - cb), pairs = cnt * w;
+            cx<T> tmp[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int idx = tid + PT * u;
+                tmp[u] = cx<T>{T(0), T(0)};     // written on every path: the array stays in registers
+                if (idx < pairs) {
+                    const int e = RM ? idx / w : idx % cnt, c = xa + cb + (RM ? idx % w : idx / cnt);
+                    tmp[u] = M[at<RM>(s_ps[e], c, np)];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int idx = tid + PT * u;
+                if (idx < pairs) {
+                    const int e = RM ? idx / w : idx % cnt, c = xa + cb + (RM ? idx % w : idx / cnt);
+                    M[at<RM>(s_pd[e], c, np)] = tmp[u];
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -1123,13 +1162,13 @@ __global__ void pad_identity_kernel(cx<T>* __restrict__ A, int n, int np, long l
 // per precision and padded size from measurements of the panel microbenchmark.
 template <class T, int S, int R, int PT>
 static void panel_go(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb,
-                     int nbatch, hipStream_t st, bool rm) {
+                     int nbatch, hipStream_t st, bool rm, int xa, int xb) {
     if (rm)
         hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, true>), dim3(nbatch), dim3(PT), 0, st,
-                           A, np, mstride, ipiv, info, pmap, k0, nb);
+                           A, np, mstride, ipiv, info, pmap, k0, nb, xa, xb);
     else
         hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, false>), dim3(nbatch), dim3(PT), 0, st, A, np, mstride,
-                           ipiv, info, pmap, k0, nb);
+                           ipiv, info, pmap, k0, nb, xa, xb);
 }
 
 // Panel shapes <S, R, PT> per size range, tried by the autotuner; variant 0 is the
@@ -1137,42 +1176,42 @@ static void panel_go(cx<T>* A, int np, long long mstride, int* ipiv, int* info, 
 // VGPRs per lane; the 512-thread ones may use 256.
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
-                  hipStream_t st, bool rm, int variant) {
+                  hipStream_t st, bool rm, int variant, int xa, int xb) {
     constexpr bool D = sizeof(T) == 8;
     if (variant < 0 || variant >= kPanelVariants) variant = 0;
     if (variant == 4) {          // 16-column sub-panels: half the passes over the panel's right part
         if (np <= 1024) {
-            if (D) panel_go<T, 16, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-            else panel_go<T, 16, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+            if (D) panel_go<T, 16, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+            else panel_go<T, 16, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
             return;
         }
         variant = 0;
     }
     if (variant == 3) {          // 256 threads: leaves room on the CU for GEMM waves of another stream
-        if (np <= 1024) panel_go<T, 8, 4, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else if (np <= 2048) panel_go<T, 4, 8, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else if (np <= 4096) panel_go<T, 2, 16, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else panel_go<T, 1, 32, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (np <= 1024) panel_go<T, 8, 4, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else if (np <= 2048) panel_go<T, 4, 8, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else if (np <= 4096) panel_go<T, 2, 16, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else panel_go<T, 1, 32, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
         return;
     }
     // up to N = 2048 the defaults are the 1024-thread shapes whose rank-S update reads the
     // L rows from LDS (at N = 1024-2048 they take 0.4-0.7 of the 512-thread shapes' time)
     if (np <= 1024) {
-        if (variant == 1) panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else if (variant == 2) panel_go<T, 4, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (variant == 1) panel_go<T, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else if (variant == 2) panel_go<T, 4, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else panel_go<T, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
     } else if (np <= 2048) {
-        if (variant == 2) panel_go<T, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else if ((variant == 0) == D) panel_go<T, 4, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (variant == 2) panel_go<T, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else if ((variant == 0) == D) panel_go<T, 4, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else panel_go<T, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
     } else if (np <= 4096) {     // complex128 default: the 512-thread shape, which does not spill
-        if (variant == 1) panel_go<T, 2, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else if ((variant == 0) == D) panel_go<T, 4, 8, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else panel_go<T, 4, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (variant == 1) panel_go<T, 2, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else if ((variant == 0) == D) panel_go<T, 4, 8, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else panel_go<T, 4, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
     } else {
-        if (variant == 1) panel_go<T, 1, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else if ((variant == 0) == D) panel_go<T, 2, 16, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
-        else panel_go<T, 2, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm);
+        if (variant == 1) panel_go<T, 1, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else if ((variant == 0) == D) panel_go<T, 2, 16, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
+        else panel_go<T, 2, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, rm, xa, xb);
     }
 }
 
@@ -1398,7 +1437,8 @@ void launch_pad_identity(cx<T>* A, int n, int np, int nbatch, hipStream_t st) {
 }
 
 #define MSG_INST(T)                                                                                                   \
-    template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t, bool, int);   \
+    template void launch_panel<T>(cx<T>*, int, long long, int*, int*, int*, int, int, int, hipStream_t, bool, int, \
+                                  int, int);                                                                       \
     template void launch_swap_trsm<T>(cx<T>*, int, long long, const int*, int, int, int, int, hipStream_t, int, bool);\
     template void launch_swap_range<T>(cx<T>*, int, long long, const int*, int, int, int, int, int, int, int, int,    \
                                        hipStream_t, int, bool, int);                                                   \

@@ -43,7 +43,9 @@ void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double
 constexpr int kPanelVariants = 5;
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
-                  hipStream_t st, bool rm = false, int variant = 0);
+                  hipStream_t st, bool rm = false, int variant = 0, int xa = 0, int xb = 0);
+// (xa, xb): columns outside the panel (at most 64) to which the panel's interchanges are
+// applied at the end of the same launch (xb <= xa: none)
 // variant: tile shape index < swap_variants() (0 = 256-column tiles, 8-column rounds)
 template <class T>
 void launch_swap_trsm(cx<T>* A, int np, long long mstride, const int* pmap, int k0, int nb, int apply_left,
