@@ -166,6 +166,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
                 const int g = c0 + tid + PT * r;
                 if (g >= j && g < np) {
                     T v = abs1(x[r][0]);
+                    if (v != v) v = T(INFINITY);    // NaN competes as infinite: the pivot is always a row in [j, np)
                     if (v > best) { best = v; bidx = g; br = r; }
                 }
             }

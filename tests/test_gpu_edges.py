@@ -78,3 +78,36 @@ def test_empty_batches(G):
     out, flags = G.tracelog_batch(med, np.zeros(0, np.complex128), G.FP64)
     assert out.size == 0 and flags.size == 0
     assert G.dk_log_det_batch(med, np.zeros(0, np.complex128), G.FP64).size == 0
+
+
+@pytest.mark.parametrize("n,nan_col", [(40, 5), (300, 130)])
+def test_nan_column_pivots_like_lapack(G, n, nan_col):
+    """A NaN column: izamax (|Re| + |Im| compared with '>') keeps the first candidate, so
+    from that column on every pivot is the diagonal row (ipiv[j] = j + 1) and info stays
+    0; the pivots before it are those of the finite part."""
+    rng = np.random.default_rng(n)
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a[:, nan_col] = np.nan
+    _, ipiv, info = G.zgetrf(a)
+    _, oipiv, _ = O.getrf(a)
+    assert info == 0
+    assert np.array_equal(ipiv, oipiv)
+    assert np.array_equal(ipiv[nan_col:], np.arange(nan_col + 1, n + 1))
+    t, _ = G.tracelog_lu(a)
+    assert np.isnan(t.real)
+
+
+def test_overflowing_entries_leave_the_batch_intact(G):
+    """At Im k = -800 the d = 3 Green function exp(ikr)/r overflows (exp(800 r)), so that
+    cell's matrix holds infinities and its factorisation NaNs; the call returns and the
+    other cells of the same batch are unaffected."""
+    pos = O.fill_medium(3, 200, 9)
+    med = G.Medium(3, 200, G.HARDSPHERE, (0.1,))
+    med.set_pos(pos)
+    ks = np.array([3.0 - 0.1j, 5.0 - 800.0j, 7.0 - 0.3j])
+    for mode, tol in ((G.FP64, 1e-10), (G.MIXED, 1e-3), (G.FP32, 1e-3)):
+        out, _ = G.tracelog_batch(med, ks, mode)
+        assert not np.isfinite(out[1])
+        for i in (0, 2):
+            want = O.tracelog(O.build_matrix(pos, ks[i], HS, (0.1,)))
+            assert abs(out[i].real - want.real) <= tol * max(1.0, abs(want.real)), (mode, ks[i])
