@@ -483,20 +483,33 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
 // The register file is rotated by one after every 4 rows, so the row being
 // eliminated is always xv[0] (forward) or xv[15] (backward) and every register
 // index is a compile-time constant while the 16-step loop stays rolled.
-template <class T, bool UNIT = true>
-__device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], const cx<T> (*sL)[65], int nb, int qq, int lane) {
+// L(row, i) from a square LDS block (row stride 65) or from packed strictly-lower storage
+// (row r at offset r (r - 1) / 2; unit diagonal only).
+template <class T>
+struct SqL {
+    const cx<T> (*s)[65];
+    __device__ __forceinline__ cx<T> operator()(int r, int c) const { return s[r][c]; }
+};
+template <class T>
+struct PackedL {
+    const cx<T>* s;
+    __device__ __forceinline__ cx<T> operator()(int r, int c) const { return s[(r * (r - 1) >> 1) + c]; }
+};
+
+template <class T, bool UNIT = true, class LA>
+__device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], LA sL, int nb, int qq, int lane) {
 #pragma unroll 1
     for (int ib = 0; ib < 16; ib++) {
 #pragma unroll
         for (int iq = 0; iq < 4; iq++) {
             const int i = 4 * ib + iq;
-            if (!UNIT && qq == iq && i < nb) xv[0] = xv[0] / sL[i][i];
+            if (!UNIT && qq == iq && i < nb) xv[0] = xv[0] / sL(i, i);
             const int src = (lane & ~3) | iq;
             const cx<T> xi{__shfl(xv[0].x, src, 64), __shfl(xv[0].y, src, 64)};
 #pragma unroll
             for (int m = 0; m < 16; m++) {
                 const int row = 4 * (m + ib) + qq;
-                if (m + ib < 16 && row > i && row < nb) xv[m] = xv[m] - sL[row][i] * xi;
+                if (m + ib < 16 && row > i && row < nb) xv[m] = xv[m] - sL(row, i) * xi;
             }
         }
         const cx<T> t = xv[0];
@@ -543,7 +556,7 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
     constexpr int U = (128 * RW + 255) / 256;
     __shared__ int s_dst[256], s_src[256];
     __shared__ int s_d1[128], s_s1[128], s_d2[128], s_s2[128], s_wc[4];
-    __shared__ cx<T> sL[64][65];
+    __shared__ cx<T> sL[64 * 63 / 2];          // strictly lower part of the unit-lower L11, packed
     const int b = blockIdx.y;
     cx<T>* M = A + (size_t)b * mstride;
     const int nq = np / nb, q = k0 / nb;
@@ -601,7 +614,7 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
         // unit-lower L11 (nb x nb) of the panel
         for (int e = tid; e < nb * nb; e += 256) {
             const int i = RM ? e / nb : e % nb, m = RM ? e % nb : e / nb;
-            sL[i][m] = M[at<RM>(k0 + i, k0 + m, np)];
+            if (i > m) sL[(i * (i - 1) >> 1) + m] = M[at<RM>(k0 + i, k0 + m, np)];
         }
     }
     __syncthreads();
@@ -645,7 +658,7 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
             const int i = 4 * m + qq;
             xv[m] = (live && i < nb) ? M[at<RM>(k0 + i, col, np)] : cx<T>{T(0), T(0)};
         }
-        lower_unit_solve<T>(xv, sL, nb, qq, lane);
+        lower_unit_solve<T>(xv, PackedL<T>{sL}, nb, qq, lane);
         if (live) {
 #pragma unroll
             for (int m = 0; m < 16; m++) {
@@ -685,7 +698,7 @@ __global__ __launch_bounds__(256) void block_w_kernel(const cx<T>* __restrict__ 
     cx<T> xv[16];
 #pragma unroll
     for (int m = 0; m < 16; m++) xv[m] = (4 * m + qq == cl) ? cx<T>{T(1), T(0)} : cx<T>{T(0), T(0)};
-    lower_unit_solve<T>(xv, sL, 64, qq, lane);       // column cl of L11^-1
+    lower_unit_solve<T>(xv, SqL<T>{sL}, 64, qq, lane);       // column cl of L11^-1
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 16; m++) sL[4 * m + qq][cl] = xv[m];
@@ -1067,7 +1080,7 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(const cx<T>* __restrict
         const int i = 4 * m + qq;
         xv[m] = live ? X[(size_t)(j0 + i) + (size_t)col * np] : cx<T>{T(0), T(0)};
     }
-    if (!UPPER) lower_unit_solve<T, UNIT>(xv, sL, 64, qq, lane);
+    if (!UPPER) lower_unit_solve<T, UNIT>(xv, SqL<T>{sL}, 64, qq, lane);
     else upper_solve<T, UNIT>(xv, sL, qq, lane);
     if (live) {
 #pragma unroll
