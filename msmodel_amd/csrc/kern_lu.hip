@@ -129,12 +129,12 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
     __shared__ int s_piv[64];
     __shared__ int s_sd[2 * S], s_ss[2 * S], s_scnt;
     __shared__ int s_pd[128], s_ps[128], s_pcnt;        // the panel's composite map, for columns [xa, xb)
-    // L rows of the current sub-panel for the LDS-broadcast rank-S update: for the
-    // 1024-thread shapes (one workgroup per CU anyway), when they fit beside the arrays
-    // above within the 160 KB of LDS.  Smaller shapes keep their LDS small so that
-    // several matrices share a CU.
-    constexpr bool LDSU = PT == 1024 && (size_t)PT * R * S * sizeof(cx<T>) <= 139264;
-    __shared__ cx<T> sL[LDSU ? S : 1][LDSU ? PT * R : 1];
+    // L rows of the current sub-panel for the LDS-broadcast rank-S update, PT rows at a
+    // time: for the 1024-thread shapes and the 512-thread 16-column ones (one workgroup
+    // per CU), when they fit beside the arrays above within the 160 KB of LDS.  Smaller
+    // shapes keep their LDS small so that several matrices share a CU.
+    constexpr bool LDSU = (PT == 1024 || (PT == 512 && S == 16)) && (size_t)PT * S * sizeof(cx<T>) <= 139264;
+    __shared__ cx<T> sL[LDSU ? S : 1][LDSU ? PT : 1];
 
     const int b = blockIdx.x;
     cx<T>* M = A + (size_t)b * mstride;
@@ -320,59 +320,62 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             }
             __syncthreads();
             if constexpr (LDSU) {
-                // rank-S update with the sub-panel's L rows broadcast through LDS
+                // rank-S update with the sub-panel's L rows broadcast through LDS, one
+                // register row (PT rows of the panel) at a time
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    const int gl = tid + PT * r;
+                    // rows c0 + [lo, hi) are those of register row r below the sub-panel
+                    const int lo = max(PT * r, S), hi = min(PT * (r + 1), np - c0);
+                    if (lo >= hi) continue;                 // uniform
 #pragma unroll
-                    for (int m = 0; m < S; m++) sL[m][gl] = x[r][m];
-                }
-                __syncthreads();
-                const int nrows = np - c0 - S;
-                if constexpr (RM) {
-                    // one column per thread (U column in registers), rows strided by the
-                    // number of row groups, UR rows in flight; consecutive threads take
-                    // consecutive columns of a row: contiguous in row-major storage
-                    constexpr int UR = 8;
-                    const int rg = PT / nright, ci = tid % nright, rq = tid / nright;
-                    if (rq < rg) {
-                        cx<T> u[S];
+                    for (int m = 0; m < S; m++) sL[m][tid] = x[r][m];
+                    __syncthreads();
+                    const int base = PT * r;
+                    if constexpr (RM) {
+                        // one column per thread (U column in registers), rows strided by the
+                        // number of row groups, UR rows in flight; consecutive threads take
+                        // consecutive columns of a row: contiguous in row-major storage
+                        constexpr int UR = 8;
+                        const int rg = PT / nright, ci = tid % nright, rq = tid / nright;
+                        if (rq < rg) {
+                            cx<T> u[S];
 #pragma unroll
-                        for (int m = 0; m < S; m++) u[m] = s_U[m][ci];
-                        for (int rb = rq; rb < nrows; rb += rg * UR) {
-                            cx<T> a[UR];
+                            for (int m = 0; m < S; m++) u[m] = s_U[m][ci];
+                            for (int rb = lo + rq; rb < hi; rb += rg * UR) {
+                                cx<T> a[UR];
 #pragma unroll
-                            for (int t = 0; t < UR; t++) {
-                                const int ri = rb + t * rg;
-                                a[t] = cx<T>{T(0), T(0)};
-                                if (ri < nrows) a[t] = M[at<true>(c0 + S + ri, c0 + S + ci, np)];
-                            }
+                                for (int t = 0; t < UR; t++) {
+                                    const int ri = rb + t * rg;
+                                    a[t] = cx<T>{T(0), T(0)};
+                                    if (ri < hi) a[t] = M[at<true>(c0 + ri, c0 + S + ci, np)];
+                                }
 #pragma unroll
-                            for (int t = 0; t < UR; t++) {
-                                const int ri = rb + t * rg;
-                                if (ri < nrows) {
-                                    cx<T> v = a[t];
+                                for (int t = 0; t < UR; t++) {
+                                    const int ri = rb + t * rg;
+                                    if (ri < hi) {
+                                        cx<T> v = a[t];
 #pragma unroll
-                                    for (int m = 0; m < S; m++) v = v - sL[m][S + ri] * u[m];
-                                    M[at<true>(c0 + S + ri, c0 + S + ci, np)] = v;
+                                        for (int m = 0; m < S; m++) v = v - sL[m][ri - base] * u[m];
+                                        M[at<true>(c0 + ri, c0 + S + ci, np)] = v;
+                                    }
                                 }
                             }
                         }
-                    }
-                } else {
-                    // every thread updates whole (row, column) elements in storage order
-                    const int tot = nrows * nright;
+                    } else {
+                        // every thread updates whole (row, column) elements in storage order
+                        const int nr = hi - lo, tot = nr * nright;
 #pragma unroll 2
-                    for (int e = tid; e < tot; e += PT) {
-                        const int ri = e % nrows, ci = e / nrows;
-                        const size_t at_e = at<false>(c0 + S + ri, c0 + S + ci, np);
-                        cx<T> a = M[at_e];
+                        for (int e = tid; e < tot; e += PT) {
+                            const int ri = lo + e % nr, ci = e / nr;
+                            const size_t at_e = at<false>(c0 + ri, c0 + S + ci, np);
+                            cx<T> a = M[at_e];
 #pragma unroll
-                        for (int m = 0; m < S; m++) a = a - sL[m][S + ri] * s_U[m][ci];
-                        M[at_e] = a;
+                            for (int m = 0; m < S; m++) a = a - sL[m][ri - base] * s_U[m][ci];
+                            M[at_e] = a;
+                        }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
                 continue;
             }
             // rank-S update of rows below the sub-panel, WR columns per round (the loads of a

@@ -8,6 +8,8 @@ import os
 import numpy as np
 import pytest
 
+import pyoracle as O
+
 pytestmark = pytest.mark.gpu
 
 
@@ -90,3 +92,32 @@ def test_graph_replays_match_eager(G, tmp_path, prec_mode):
             assert np.array_equal(got, want), rep
             G.dk_log_det_batch(other, ks[:3], G.FP64)          # eager work between replays
     table(0)
+
+
+@pytest.mark.parametrize("prec_mode", ["FP64", "FP32"])
+@pytest.mark.parametrize("n,np_,variants", [(1000, 1024, (0, 1, 2, 3, 4)), (1100, 1152, (0, 1, 2, 3))])
+def test_panel_shapes_beyond_one_register_row(G, tmp_path, prec_mode, n, np_, variants):
+    """Panel shapes whose threads hold several rows of the panel (Np above the thread
+    count: the rank-S update stages one register row at a time): every shape gives the
+    same trace-log, row-major with 128-column blocks, and one cell matches the oracle."""
+    mode = getattr(G, prec_mode)
+    prec = 0 if mode == G.FP64 else 1
+    pos = O.fill_medium(3, n, 12)
+    med = G.Medium(3, n, G.HARDSPHERE, (0.05,))
+    med.set_pos(pos)
+    ks = np.array([6.0 - 0.3j, 14.0 - 1.1j, 22.0 - 0.05j])
+    outs = []
+    for pv in variants:
+        path = os.path.join(str(tmp_path), "panel_%d.txt" % pv)
+        with open(path, "w") as f:
+            f.write("%d %d 4 64 1 1 %d 0 1\n" % (np_, prec, pv))
+        assert G.tune_load(path) == 1
+        outs.append(G.tracelog_batch(med, ks, mode)[0])
+    tol = 1e-12 if prec == 0 else 1e-5
+    for pv, out in zip(variants, outs):
+        assert np.abs(out.real - outs[0].real).max() <= tol * np.abs(outs[0].real).max(), pv
+        dim = (n * (out.imag - outs[0].imag) + np.pi) % (2 * np.pi) - np.pi
+        assert np.abs(dim).max() <= (1e-9 if prec == 0 else 1e-2), pv
+    want = O.tracelog(O.build_matrix(pos, ks[1], O.HARDSPHERE, (0.05,)))
+    assert abs(outs[-1][1].real - want.real) <= (1e-10 if prec == 0 else 1e-3) * abs(want.real)
+    G.tune_load(G.tuning_table_path())
