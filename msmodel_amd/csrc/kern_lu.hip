@@ -130,10 +130,10 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
     __shared__ int s_sd[2 * S], s_ss[2 * S], s_scnt;
     __shared__ int s_pd[128], s_ps[128], s_pcnt;        // the panel's composite map, for columns [xa, xb)
     // L rows of the current sub-panel for the LDS-broadcast rank-S update, PT rows at a
-    // time: for the 1024-thread shapes and the 512-thread 16-column ones (one workgroup
-    // per CU), when they fit beside the arrays above within the 160 KB of LDS.  Smaller
-    // shapes keep their LDS small so that several matrices share a CU.
-    constexpr bool LDSU = (PT == 1024 || (PT == 512 && S == 16)) && (size_t)PT * S * sizeof(cx<T>) <= 139264;
+    // time, whenever they fit beside the arrays above within the 160 KB of LDS.  The
+    // 256-thread shapes then need 45 KB or less and half the register file, which leaves
+    // room on the CU for the GEMM workgroups of other streams.
+    constexpr bool LDSU = (size_t)PT * S * sizeof(cx<T>) <= 139264;
     __shared__ cx<T> sL[LDSU ? S : 1][LDSU ? PT : 1];
 
     const int b = blockIdx.x;
