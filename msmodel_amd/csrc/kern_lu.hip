@@ -678,49 +678,47 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
 // lower block of k1 and L21 = M(k1:ke, K0:k1).  That is one update of K = 128,
 //   C <- C - Wb [U12(K0); C],   Wb = [L11^-1 L21, I - L11^-1]   (64 x 128),
 // on the rows K0..ke of the trailing columns.  One 256-thread workgroup per matrix forms
-// Wb: L11^-1 by forward substitution of the identity (4 lanes per column), then
-// L11^-1 L21.  Wb is stored column-major (ld 64) for column-major factors and row-major
-// (row stride 128) for row-major ones, where the update is applied to the transposed
-// blocks and Wb^T is the right operand.
+// Wb by forward substitution (4 lanes per column) against L11 packed in LDS (32 KB, so
+// that the workgroup shares a CU with others): L11^-1 L21 from the columns of L21, then
+// L11^-1 from those of the identity.  Wb is stored column-major (ld 64) for column-major
+// factors and row-major (row stride 128) for row-major ones, where the update is applied
+// to the transposed blocks and Wb^T is the right operand.
 template <class T, bool RM>
 __global__ __launch_bounds__(256) void block_w_kernel(const cx<T>* __restrict__ A, int np, long long mstride, int K0,
                                                       cx<T>* __restrict__ W) {
-    __shared__ cx<T> sL[64][65];
-    __shared__ cx<T> sB[64][65];
+    __shared__ cx<T> sL[64 * 63 / 2];          // strictly lower part of the unit-lower L11, packed
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const cx<T>* M = A + (size_t)b * mstride;
     cx<T>* Wb = W + (size_t)b * (64 * 128);
     const int k1 = K0 + 64;
     for (int e = tid; e < 64 * 64; e += 256) {
         const int i = RM ? e / 64 : e % 64, m = RM ? e % 64 : e / 64;
-        sL[i][m] = M[at<RM>(k1 + i, k1 + m, np)];
-        sB[i][m] = M[at<RM>(k1 + i, K0 + m, np)];
+        if (i > m) sL[(i * (i - 1) >> 1) + m] = M[at<RM>(k1 + i, k1 + m, np)];
     }
     __syncthreads();
+    // thread: column cl of the right-hand sides, rows 4 m + qq
     const int cl = tid >> 2, qq = tid & 3;
     cx<T> xv[16];
+    // Wb(:, cl) = L11^-1 L21(:, cl), with L21 = M(k1:ke, K0:k1)
+#pragma unroll
+    for (int m = 0; m < 16; m++) xv[m] = M[at<RM>(k1 + 4 * m + qq, K0 + cl, np)];
+    lower_unit_solve<T>(xv, PackedL<T>{sL}, 64, qq, lane);
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        const int i = 4 * m + qq;
+        if (RM) Wb[i * 128 + cl] = xv[m];
+        else Wb[i + cl * 64] = xv[m];
+    }
+    // Wb(:, 64 + cl) = (I - L11^-1)(:, cl)
 #pragma unroll
     for (int m = 0; m < 16; m++) xv[m] = (4 * m + qq == cl) ? cx<T>{T(1), T(0)} : cx<T>{T(0), T(0)};
-    lower_unit_solve<T>(xv, SqL<T>{sL}, 64, qq, lane);       // column cl of L11^-1
-    __syncthreads();
+    lower_unit_solve<T>(xv, PackedL<T>{sL}, 64, qq, lane);
 #pragma unroll
-    for (int m = 0; m < 16; m++) sL[4 * m + qq][cl] = xv[m];
-    __syncthreads();
-    // thread: row i = tid / 4, columns p = tid % 4 + 4 t
-    const int i = tid >> 2;
-#pragma unroll 4
-    for (int t = 0; t < 16; t++) {
-        const int p = (tid & 3) + 4 * t;
-        cx<T> acc{T(0), T(0)};
-        for (int j = 0; j <= i; j++) acc = acc + sL[i][j] * sB[j][p];
-        const cx<T> ip = (p == i) ? cx<T>{T(0), T(0)} : cx<T>{-sL[i][p].x, -sL[i][p].y};   // (I - L11^-1)[i][p]
-        if (RM) {
-            Wb[i * 128 + p] = acc;
-            Wb[i * 128 + 64 + p] = ip;
-        } else {
-            Wb[i + p * 64] = acc;
-            Wb[i + (64 + p) * 64] = ip;
-        }
+    for (int m = 0; m < 16; m++) {
+        const int i = 4 * m + qq;
+        const cx<T> v = (i == cl) ? cx<T>{T(0), T(0)} : cx<T>{-xv[m].x, -xv[m].y};
+        if (RM) Wb[i * 128 + 64 + cl] = v;
+        else Wb[i + (64 + cl) * 64] = v;
     }
 }
 
