@@ -288,10 +288,27 @@ struct Tune {
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, int>, Tune> g_tune;
 
+// A size without an entry takes that of the nearest size of the same precision (by
+// ratio; the larger on a tie), without graph replay (a capture per size would not pay
+// off for sizes met a few times); with no entry of that precision, the built-in shapes.
 static Tune tune_for(int np, int prec) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find({np, prec});
-    return it == g_tune.end() ? Tune{} : it->second;
+    if (it != g_tune.end()) return it->second;
+    const Tune* best = nullptr;
+    double bd = 0.0;
+    for (const auto& kv : g_tune) {
+        if (kv.first.second != prec) continue;
+        const double d = std::fabs(std::log((double)kv.first.first / np));
+        if (!best || d < bd || (d == bd && kv.first.first > np)) {
+            best = &kv.second;
+            bd = d;
+        }
+    }
+    if (!best) return Tune{};
+    Tune t = *best;
+    t.graph = 0;
+    return t;
 }
 template <class T> static int prec_of() { return sizeof(T) == 8 ? 0 : 1; }
 
