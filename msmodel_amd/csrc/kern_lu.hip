@@ -486,17 +486,23 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
 // The register file is rotated by one after every 4 rows, so the row being
 // eliminated is always xv[0] (forward) or xv[15] (backward) and every register
 // index is a compile-time constant while the 16-step loop stays rolled.
-// L(row, i) from a square LDS block (row stride 65) or from packed strictly-lower storage
-// (row r at offset r (r - 1) / 2; unit diagonal only).
-template <class T>
-struct SqL {
-    const cx<T> (*s)[65];
-    __device__ __forceinline__ cx<T> operator()(int r, int c) const { return s[r][c]; }
-};
+// L(row, i) from packed strictly-lower storage (row r at offset r (r - 1) / 2; unit
+// diagonal only).
 template <class T>
 struct PackedL {
     const cx<T>* s;
     __device__ __forceinline__ cx<T> operator()(int r, int c) const { return s[(r * (r - 1) >> 1) + c]; }
+};
+// Packed triangles with their diagonal: lower (c <= r) by rows, upper (r <= c) by columns.
+template <class T>
+struct PackedLD {
+    const cx<T>* s;
+    __device__ __forceinline__ cx<T> operator()(int r, int c) const { return s[(r * (r + 1) >> 1) + c]; }
+};
+template <class T>
+struct PackedUD {
+    const cx<T>* s;
+    __device__ __forceinline__ cx<T> operator()(int r, int c) const { return s[(c * (c + 1) >> 1) + r]; }
 };
 
 template <class T, bool UNIT = true, class LA>
@@ -522,20 +528,20 @@ __device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], LA sL, int nb,
     }
 }
 
-template <class T, bool UNIT = false>
-__device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], const cx<T> (*sL)[65], int qq, int lane) {
+template <class T, bool UNIT = false, class LA>
+__device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], LA sL, int qq, int lane) {
 #pragma unroll 1
     for (int ib = 15; ib >= 0; ib--) {
 #pragma unroll
         for (int iq = 3; iq >= 0; iq--) {
             const int i = 4 * ib + iq;
-            if (!UNIT && qq == iq) xv[15] = xv[15] / sL[i][i];
+            if (!UNIT && qq == iq) xv[15] = xv[15] / sL(i, i);
             const int src = (lane & ~3) | iq;
             const cx<T> xi{__shfl(xv[15].x, src, 64), __shfl(xv[15].y, src, 64)};
 #pragma unroll
             for (int m = 0; m < 16; m++) {
                 const int row = 4 * (m - (15 - ib)) + qq;
-                if (m >= 15 - ib && row < i) xv[m] = xv[m] - sL[row][i] * xi;
+                if (m >= 15 - ib && row < i) xv[m] = xv[m] - sL(row, i) * xi;
             }
         }
         const cx<T> t = xv[15];
@@ -1063,13 +1069,14 @@ __global__ void gather_rows_kernel(cx<T>* __restrict__ dst, const cx<T>* __restr
 template <class T, int UPPER, bool UNIT>
 __global__ __launch_bounds__(256) void trsm_block_kernel(const cx<T>* __restrict__ LU, cx<T>* __restrict__ B,
                                                          int np, int ncol, int j0, long long slu, long long sb) {
-    __shared__ cx<T> sL[64][65];
+    __shared__ cx<T> sL[64 * 65 / 2];          // the triangle with its diagonal, packed
     const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
     const cx<T>* F = LU + (size_t)b * slu;
     cx<T>* X = B + (size_t)b * sb;
     for (int e = tid; e < 64 * 64; e += 256) {
         const int i = e % 64, m = e / 64;
-        sL[i][m] = F[(size_t)(j0 + i) + (size_t)(j0 + m) * np];
+        if (UPPER ? i <= m : i >= m)
+            sL[UPPER ? (m * (m + 1) >> 1) + i : (i * (i + 1) >> 1) + m] = F[(size_t)(j0 + i) + (size_t)(j0 + m) * np];
     }
     __syncthreads();
     const int cl = tid >> 2, qq = tid & 3;
@@ -1081,8 +1088,8 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(const cx<T>* __restrict
         const int i = 4 * m + qq;
         xv[m] = live ? X[(size_t)(j0 + i) + (size_t)col * np] : cx<T>{T(0), T(0)};
     }
-    if (!UPPER) lower_unit_solve<T, UNIT>(xv, SqL<T>{sL}, 64, qq, lane);
-    else upper_solve<T, UNIT>(xv, sL, qq, lane);
+    if (!UPPER) lower_unit_solve<T, UNIT>(xv, PackedLD<T>{sL}, 64, qq, lane);
+    else upper_solve<T, UNIT>(xv, PackedUD<T>{sL}, qq, lane);
     if (live) {
 #pragma unroll
         for (int m = 0; m < 16; m++) X[(size_t)(j0 + 4 * m + qq) + (size_t)col * np] = xv[m];
