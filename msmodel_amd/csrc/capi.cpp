@@ -418,7 +418,8 @@ MSGPU_API msgpu_cplx tracelog_lu(int n, msgpu_cplx* matrix) {
     std::vector<int> ipiv(n);
     cx<double> tl{0, 0};
     int info = 0;
-    die_on(lu_host(n, as_cx(matrix), n, ipiv.data(), 0, nullptr, 0, &tl, &info), "tracelog_lu");
+    // concurrent calls of the caller's threads share one batched factorization
+    die_on(tracelog_host_coalesced(n, as_cx(matrix), n, ipiv.data(), &tl, &info), "tracelog_lu");
     if (info > 0) printf("[LAPACK] Element U(%d,%d) is exactly zero, the determinant will be singular.\n", info, info);
     return to_c(tl);
 }

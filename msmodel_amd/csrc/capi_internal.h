@@ -83,6 +83,9 @@ int incident_host(const MediumView& mv, cx<double> k, int source, cx<double>* ou
 // Optionally solves nrhs right-hand sides b (ld ldb) in place and returns the tracelog.
 int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, int ldb, cx<double>* tracelog,
             int* info);
+// tracelog_lu of one host matrix (n x n, ld lda; overwritten by its LAPACK-format
+// factor), coalesced with the calls of other threads into one batched factorization.
+int tracelog_host_coalesced(int n, cx<double>* a, int lda, int* ipiv, cx<double>* tracelog, int* info);
 // Solve with an existing LAPACK-format LU from the host (zgetrs 'N').
 int lu_solve_host(int n, const cx<double>* lu, int lda, const int* ipiv, int nrhs, cx<double>* b, int ldb);
 // Autotuner (see engine.cpp): report = {swap variant, chunk, streams, best ms, row-major,

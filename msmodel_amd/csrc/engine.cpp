@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -42,12 +43,17 @@ struct HipFail : std::runtime_error {
 
 // ------------------------------------------------------------------ device pool
 // Buffers are recycled by size class (next power of two above 1 MiB granularity)
-// and never returned to the driver, so steady-state calls allocate nothing.
+// and never returned to the driver, so steady-state calls allocate nothing.  The same
+// pool type holds the pinned host buffers of the host-buffer services.
 class Pool {
     std::mutex mu_;
     std::multimap<size_t, void*> free_;
+    const bool pinned_;
+    hipError_t raw_alloc(void** p, size_t b) { return pinned_ ? hipHostMalloc(p, b, hipHostMallocDefault) : hipMalloc(p, b); }
+    void raw_free(void* p) { (void)(pinned_ ? hipHostFree(p) : hipFree(p)); }
 
 public:
+    explicit Pool(bool pinned = false) : pinned_(pinned) {}
     static size_t klass(size_t b) {
         size_t c = (size_t)1 << 16;
         while (c < b) c <<= 1;
@@ -65,13 +71,13 @@ public:
             }
         }
         void* p = nullptr;
-        hipError_t e = hipMalloc(&p, cls);
+        hipError_t e = raw_alloc(&p, cls);
         if (e != hipSuccess) {
             // release cached buffers and retry once
             std::lock_guard<std::mutex> g(mu_);
-            for (auto& kv : free_) (void)hipFree(kv.second);
+            for (auto& kv : free_) raw_free(kv.second);
             free_.clear();
-            HIPCHK(hipMalloc(&p, cls));
+            HIPCHK(raw_alloc(&p, cls));
         }
         return p;
     }
@@ -83,6 +89,10 @@ public:
 
 static Pool& pool() {
     static Pool* p = new Pool();
+    return *p;
+}
+static Pool& pinned_pool() {
+    static Pool* p = new Pool(true);
     return *p;
 }
 
@@ -432,7 +442,10 @@ static void factor2(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* 
             swap(K0, K0, ke, np, 1, 1, k1);       // interchanges of K0 and k1 composed, U12 of K0
             // U12 of k1 = L11^-1 (A - L21 U12(K0)) as one K = 128 update with Wb
             launch_block_w<T>(mats, np, ms, K0, wblk, nbatch, st, rm);
-            timed(2, 8.0 * nbatch * (double)NB * (np - ke) * 2 * NB, es * nbatch * (2.0 * NB * (np - ke) + (NB + np - ke) * 2.0 * NB), st, [&] {
+            // algorithmic flops: L21 U12(K0) (NB per column and row) and the unit-lower
+            // solve (NB / 2); the other NB / 2 multiply the zero upper part of I - L11^-1
+            timed(2, 8.0 * nbatch * (double)NB * (np - ke) * 1.5 * NB,
+                  es * nbatch * (2.0 * NB * (np - ke) + (NB + np - ke) * 2.0 * NB), st, [&] {
                 if (rm)
                     launch_gemm_sub<T>(mats + ke + (size_t)k1 * np, mats + ke + (size_t)K0 * np, wblk, np - ke, NB,
                                        2 * NB, np, 2 * NB, np, ms, ms, 64 * 128, nbatch, st);
@@ -1262,6 +1275,47 @@ int wave_points(const MediumView& mv, cx<double> k, int source, const cx<double>
 }
 
 // ------------------------------------------------------------------ host-buffer matrix services
+// Host (pageable) column-major blocks move through a pinned buffer of the call: the
+// calling thread packs or unpacks the block with memcpy, and the device copies the
+// packed block at the link's rate.  (A pitched copy straight from pageable memory is
+// staged by the runtime and serializes the threads that call concurrently.)
+struct HostStage {
+    cx<double>* p = nullptr;
+    size_t cls = 0;
+    explicit HostStage(size_t count) { p = (cx<double>*)pinned_pool().get(sizeof(cx<double>) * count, cls); }
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    ~HostStage() { pinned_pool().put(p, cls); }
+};
+// stage <- host src (ld lds), rows x cols, packed (ld rows)
+static void pack_block(const HostStage& hs, const cx<double>* src, int lds, int rows, int cols) {
+    const size_t rb = sizeof(cx<double>) * rows;
+    if (lds == rows) memcpy(hs.p, src, rb * cols);
+    else for (int j = 0; j < cols; j++) memcpy(hs.p + (size_t)j * rows, src + (size_t)j * lds, rb);
+}
+// device dst (ld ldd) <- the packed stage
+static void h2d_packed(cx<double>* dst, int ldd, int rows, int cols, const HostStage& hs, hipStream_t st) {
+    const size_t rb = sizeof(cx<double>) * rows;
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(cx<double>) * ldd, hs.p, rb, rb, cols, hipMemcpyHostToDevice, st));
+}
+// device dst (ld ldd) <- host src (ld lds), rows x cols
+static void h2d_block(cx<double>* dst, int ldd, const cx<double>* src, int lds, int rows, int cols,
+                      const HostStage& hs, hipStream_t st) {
+    pack_block(hs, src, lds, rows, cols);
+    h2d_packed(dst, ldd, rows, cols, hs, st);
+}
+// host dst (ld ldd) <- device src (ld lds): the copy into the stage, then (after the
+// stream is synchronized) its unpacking
+static void d2h_block_copy(const cx<double>* src, int lds, int rows, int cols, const HostStage& hs, hipStream_t st) {
+    const size_t rb = sizeof(cx<double>) * rows;
+    HIPCHK(hipMemcpy2DAsync(hs.p, rb, src, sizeof(cx<double>) * lds, rb, cols, hipMemcpyDeviceToHost, st));
+}
+static void unpack_block(cx<double>* dst, int ldd, int rows, int cols, const HostStage& hs) {
+    const size_t rb = sizeof(cx<double>) * rows;
+    if (ldd == rows) memcpy(dst, hs.p, rb * cols);
+    else for (int j = 0; j < cols; j++) memcpy(dst + (size_t)j * ldd, hs.p + (size_t)j * rows, rb);
+}
+
 int build_matrix_host(const MediumView& mv, cx<double> k, int kind, cx<double>* out) {
     if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
     return guarded([&]() -> int {
@@ -1269,14 +1323,15 @@ int build_matrix_host(const MediumView& mv, cx<double> k, int kind, cx<double>* 
         const int np = mv.np, n = mv.n;
         DBuf<double> dpos((size_t)n * mv.d);
         DBuf<cx<double>> dk(1), ddiag(1), diz(1), m((size_t)np * np);
+        HostStage hs((size_t)n * n);
         HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpyAsync(dk.p, &k, sizeof(k), hipMemcpyHostToDevice, L.st));
         launch_diag(dk.p, 1, mv.m, mv.d, kind, ddiag.p, diz.p, L.st);
         launch_assemble<double>(dpos.p, n, mv.d, np, dk.p, ddiag.p, diz.p, kind, m.p, (long long)np * np, 1, L.st);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpy2DAsync(out, sizeof(cx<double>) * n, m.p, sizeof(cx<double>) * np, sizeof(cx<double>) * n, n,
-                                hipMemcpyDeviceToHost, L.st));
+        d2h_block_copy(m.p, np, n, n, hs, L.st);
         L.sync();
+        unpack_block(out, n, n, n, hs);
         return 0;
     });
 }
@@ -1313,10 +1368,10 @@ int plane_wave_host(const MediumView& mv, cx<double> k, double theta_deg, cx<dou
 }
 
 // Host matrix -> padded device matrix (identity padding).
-static void upload_padded(cx<double>* dm, int n, int np, const cx<double>* a, int lda, hipStream_t st) {
+static void upload_padded(cx<double>* dm, int n, int np, const cx<double>* a, int lda, const HostStage& hs,
+                          hipStream_t st) {
     HIPCHK(hipMemsetAsync(dm, 0, sizeof(cx<double>) * (size_t)np * np, st));
-    HIPCHK(hipMemcpy2DAsync(dm, sizeof(cx<double>) * np, a, sizeof(cx<double>) * lda, sizeof(cx<double>) * n, n,
-                            hipMemcpyHostToDevice, st));
+    h2d_block(dm, np, a, lda, n, n, hs, st);
     launch_pad_identity<double>(dm, n, np, 1, st);
 }
 
@@ -1327,11 +1382,11 @@ int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, i
         const int np = padded(n);
         DBuf<cx<double>> m((size_t)np * np), x((size_t)np * (nrhs > 0 ? nrhs : 1)), tl(1);
         DBuf<int> dpiv(np), dinfo(1), pmap(factor_ws_ints(np, NB, 1));
-        upload_padded(m.p, n, np, a, lda, L.st);
+        HostStage ha((size_t)n * n), hb((size_t)n * (nrhs > 0 ? nrhs : 1));
+        upload_padded(m.p, n, np, a, lda, ha, L.st);
         if (nrhs > 0) {
             HIPCHK(hipMemsetAsync(x.p, 0, sizeof(cx<double>) * (size_t)np * nrhs, L.st));
-            HIPCHK(hipMemcpy2DAsync(x.p, sizeof(cx<double>) * np, b, sizeof(cx<double>) * ldb, sizeof(cx<double>) * n,
-                                    nrhs, hipMemcpyHostToDevice, L.st));
+            h2d_block(x.p, np, b, ldb, n, nrhs, hb, L.st);
         }
         // factor without left interchanges, solve, then apply them for the LAPACK-format output
         factor<double>(m.p, np, 1, dpiv.p, dinfo.p, pmap.p, false, false, L.st);
@@ -1339,15 +1394,125 @@ int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, i
         if (nrhs > 0) lu_solve<double>(m.p, x.p, np, nrhs, 1, pmap.p, np, L.st);
         for (int k0 = NB; k0 < np; k0 += NB) launch_swap_left<double>(m.p, np, (long long)np * np, pmap.p, k0, NB, 1, L.st);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpy2DAsync(a, sizeof(cx<double>) * lda, m.p, sizeof(cx<double>) * np, sizeof(cx<double>) * n, n,
-                                hipMemcpyDeviceToHost, L.st));
+        d2h_block_copy(m.p, np, n, n, ha, L.st);      // the upload from ha precedes it on the stream
         HIPCHK(hipMemcpyAsync(ipiv, dpiv.p, sizeof(int) * n, hipMemcpyDeviceToHost, L.st));
         HIPCHK(hipMemcpyAsync(info, dinfo.p, sizeof(int), hipMemcpyDeviceToHost, L.st));
-        if (nrhs > 0)
-            HIPCHK(hipMemcpy2DAsync(b, sizeof(cx<double>) * ldb, x.p, sizeof(cx<double>) * np, sizeof(cx<double>) * n,
-                                    nrhs, hipMemcpyDeviceToHost, L.st));
+        if (nrhs > 0) d2h_block_copy(x.p, np, n, nrhs, hb, L.st);
         if (tracelog) HIPCHK(hipMemcpyAsync(tracelog, tl.p, sizeof(cx<double>), hipMemcpyDeviceToHost, L.st));
         L.sync();
+        unpack_block(a, lda, n, n, ha);
+        if (nrhs > 0) unpack_block(b, ldb, n, nrhs, hb);
+        return 0;
+    });
+}
+
+// ------------------------------------------------------------------ coalesced trace-logs
+// Concurrent one-matrix trace-logs (the drop-in tracelog_lu, called from the caller's
+// host threads) run as one batched factorization.  A call packs its matrix into a
+// pinned buffer and queues a request; a waiting caller, while fewer than
+// kTlBatches batches run, takes every queued request of the first one's size, runs
+// them as one batch (one upload per matrix, the batched LU, trace-logs, left
+// interchanges, one download per matrix) and marks them done; each caller unpacks its
+// own factor.  Requests that arrive while batches run form the next ones; two batches
+// at a time let one's copies overlap the other's factorization.
+namespace {
+struct TlReq {
+    int n;
+    const HostStage* hs;       // packed n x n matrix in, LAPACK-format factor out
+    int* ipiv;                  // n pivots out (host)
+    cx<double> tl{0.0, 0.0};
+    int info = 0;
+    bool done = false;
+    std::string err;
+};
+std::mutex g_tl_mu;
+std::condition_variable g_tl_cv;
+std::vector<TlReq*> g_tl_queue;
+int g_tl_active = 0;
+constexpr int kTlBatches = 2;
+
+void run_tl_batch(const std::vector<TlReq*>& B) {
+    Lease L;
+    const int n = B[0]->n, np = padded(n), nb = (int)B.size();
+    const long long ms = (long long)np * np;
+    DBuf<cx<double>> m((size_t)nb * ms), tl(nb);
+    DBuf<int> dpiv((size_t)nb * np), dinfo(nb), pmap(factor_ws_ints(np, NB, nb));
+    HIPCHK(hipMemsetAsync(m.p, 0, sizeof(cx<double>) * (size_t)nb * ms, L.st));
+    for (int i = 0; i < nb; i++) h2d_packed(m.p + (size_t)i * ms, np, n, n, *B[i]->hs, L.st);
+    launch_pad_identity<double>(m.p, n, np, nb, L.st);
+    factor<double>(m.p, np, nb, dpiv.p, dinfo.p, pmap.p, false, false, L.st);
+    launch_tracelog<double>(m.p, n, nullptr, np, ms, dpiv.p, tl.p, nullptr, nb, L.st);
+    for (int k0 = NB; k0 < np; k0 += NB) launch_swap_left<double>(m.p, np, ms, pmap.p, k0, NB, nb, L.st);
+    HIPCHK(hipGetLastError());
+    std::vector<int> hpiv((size_t)nb * np), hinfo(nb);
+    std::vector<cx<double>> htl(nb);
+    for (int i = 0; i < nb; i++) d2h_block_copy(m.p + (size_t)i * ms, np, n, n, *B[i]->hs, L.st);
+    HIPCHK(hipMemcpyAsync(hpiv.data(), dpiv.p, sizeof(int) * (size_t)nb * np, hipMemcpyDeviceToHost, L.st));
+    HIPCHK(hipMemcpyAsync(hinfo.data(), dinfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, L.st));
+    HIPCHK(hipMemcpyAsync(htl.data(), tl.p, sizeof(cx<double>) * nb, hipMemcpyDeviceToHost, L.st));
+    L.sync();
+    for (int i = 0; i < nb; i++) {
+        memcpy(B[i]->ipiv, hpiv.data() + (size_t)i * np, sizeof(int) * n);
+        B[i]->info = hinfo[i];
+        B[i]->tl = htl[i];
+    }
+}
+}  // namespace
+
+int tracelog_host_coalesced(int n, cx<double>* a, int lda, int* ipiv, cx<double>* tracelog, int* info) {
+    return guarded([&]() -> int {
+        if (n < 0) throw std::runtime_error("tracelog_lu: negative matrix size");
+        if (n > 8192) throw std::runtime_error("matrices above N = 8192 are not supported by the device LU");
+        if (n == 0) {                   // the empty sum divided by N = 0
+            *tracelog = cx<double>{NAN, NAN};
+            *info = 0;
+            return 0;
+        }
+        HostStage hs((size_t)n * n);
+        pack_block(hs, a, lda, n, n);
+        TlReq req;
+        req.n = n;
+        req.hs = &hs;
+        req.ipiv = ipiv;
+        std::unique_lock<std::mutex> lk(g_tl_mu);
+        g_tl_queue.push_back(&req);
+        while (!req.done) {
+            if (g_tl_active >= kTlBatches || g_tl_queue.empty()) {
+                g_tl_cv.wait(lk);
+                continue;
+            }
+            // lead one batch: the queued requests of the first request's size
+            g_tl_active++;
+            std::vector<TlReq*> batch;
+            const int bn = g_tl_queue.front()->n;
+            for (size_t i = 0; i < g_tl_queue.size();) {
+                if (g_tl_queue[i]->n == bn && batch.size() < 256) {
+                    batch.push_back(g_tl_queue[i]);
+                    g_tl_queue.erase(g_tl_queue.begin() + (long)i);
+                } else {
+                    i++;
+                }
+            }
+            lk.unlock();
+            std::string err;
+            try {
+                run_tl_batch(batch);
+            } catch (const std::exception& ex) {
+                err = ex.what();
+            }
+            lk.lock();
+            for (TlReq* r : batch) {
+                r->err = err;
+                r->done = true;
+            }
+            g_tl_active--;
+            g_tl_cv.notify_all();
+        }
+        lk.unlock();
+        if (!req.err.empty()) throw std::runtime_error(req.err);
+        unpack_block(a, lda, n, n, hs);
+        *tracelog = req.tl;
+        *info = req.info;
         return 0;
     });
 }
@@ -1362,10 +1527,10 @@ int lu_solve_host(int n, const cx<double>* lu, int lda, const int* ipiv, int nrh
         for (int i = 0; i < n; i++) std::swap(perm[i], perm[ipiv[i] - 1]);
         DBuf<cx<double>> m((size_t)np * np), x((size_t)np * nrhs), y((size_t)np * nrhs);
         DBuf<int> dperm(np);
-        upload_padded(m.p, n, np, lu, lda, L.st);
+        HostStage ha((size_t)n * n), hb((size_t)n * nrhs);
+        upload_padded(m.p, n, np, lu, lda, ha, L.st);
         HIPCHK(hipMemsetAsync(x.p, 0, sizeof(cx<double>) * (size_t)np * nrhs, L.st));
-        HIPCHK(hipMemcpy2DAsync(x.p, sizeof(cx<double>) * np, b, sizeof(cx<double>) * ldb, sizeof(cx<double>) * n, nrhs,
-                                hipMemcpyHostToDevice, L.st));
+        h2d_block(x.p, np, b, ldb, n, nrhs, hb, L.st);
         HIPCHK(hipMemcpyAsync(dperm.p, perm.data(), sizeof(int) * np, hipMemcpyHostToDevice, L.st));
         launch_gather_rows<double>(y.p, x.p, dperm.p, np, nrhs, 0, 0, 1, L.st);
         const long long ms = (long long)np * np;
@@ -1381,9 +1546,9 @@ int lu_solve_host(int n, const cx<double>* lu, int lda, const int* ipiv, int nrh
                 launch_gemm_sub<double>(y.p, m.p + (size_t)k0 * np, y.p + k0, k0, nrhs, NB, np, np, np, 0, 0, 0, 1, L.st);
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpy2DAsync(b, sizeof(cx<double>) * ldb, y.p, sizeof(cx<double>) * np, sizeof(cx<double>) * n, nrhs,
-                                hipMemcpyDeviceToHost, L.st));
+        d2h_block_copy(y.p, np, n, nrhs, hb, L.st);
         L.sync();
+        unpack_block(b, ldb, n, nrhs, hb);
         return 0;
     });
 }

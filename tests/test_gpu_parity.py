@@ -120,6 +120,26 @@ def test_tracelog_lu_dropin(G):
         tl_close([t], [O.tracelog(mm)], 50)
 
 
+def test_tracelog_lu_concurrent_calls(G):
+    """Drop-in tracelog_lu called from many host threads at once (the calls are coalesced
+    into shared batched factorizations, sizes mixed): every call returns what it returns
+    alone, trace-log and LAPACK-format factor bit for bit; an empty matrix gives 0/0."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(21)
+    sizes = [60, 130, 60, 200, 130, 60, 70, 200, 60, 130, 60, 60] * 2
+    mats = [rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) for n in sizes]
+    alone = [G.tracelog_lu(m) for m in mats]
+    with ThreadPoolExecutor(max_workers=12) as ex:
+        together = list(ex.map(G.tracelog_lu, mats))
+    for (t0, lu0), (t1, lu1), m in zip(alone, together, mats):
+        assert t0 == t1
+        assert np.array_equal(lu0, lu1)
+    for i in (0, 3):
+        tl_close([alone[i][0]], [O.tracelog(mats[i])], sizes[i])
+    t, _ = G.tracelog_lu(np.zeros((0, 0), np.complex128))
+    assert np.isnan(t.real)
+
+
 @pytest.mark.parametrize("tag,p,n", [("test", 0.4, 50), ("cfg1", 0.1, 200), ("cfg3", 0.05, 500),
                                      ("n1000", 0.05, 1000), ("d2", 0.1, 400)])
 def test_tracelog_batch_fp64_vs_reference(G, ref, tag, p, n):
