@@ -64,16 +64,8 @@ def measured_traffic(kernel):
     return None if rec is None else rec["read_bytes_per_launch"] + rec["write_bytes_per_launch"]
 
 
-def cpu_baseline(n, nx, ny):
-    """Times the reference msmain (oracle/_ref, built from the MSModel sources with MKL) on a k-map sample."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "msmain")
-    if not os.path.exists(exe):
-        return None
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+def run_msmain(exe, n, nx, ny, threads):
+    """Runs an msmain build on a d=3 unit-density k-map config; returns its realtime (s)."""
     tmp = tempfile.mkdtemp(prefix="msbench_")
     try:
         conf = os.path.join(tmp, "map.conf")
@@ -81,18 +73,36 @@ def cpu_baseline(n, nx, ny):
             f.write("[medium]\ndimension=3\nnatom=%d\nmodel=hardsphere 0.05\nshape=ball\nratio=1\n\n"
                     "[kplane]\nnseed=1\nxrange=5:30\nyrange=-2:0\nxsample=%d\nysample=%d\n"
                     "color=temperature div 5\ntitle=bench\n" % (n, nx, ny))
-        env = dict(os.environ, MKL_THREADING_LAYER="GNU", MKL_NUM_THREADS="1", OMP_NUM_THREADS=str(cores))
-        subprocess.run([exe, "-p", str(cores), conf], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL,
+        env = dict(os.environ, MKL_THREADING_LAYER="GNU", MKL_NUM_THREADS="1", OMP_NUM_THREADS=str(threads))
+        subprocess.run([exe, "-p", str(threads), conf], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL, timeout=300)
-        dat = None
         for dp, _, files in os.walk(os.path.join(tmp, "out")):
             for fn in files:
                 if fn.endswith(".dat"):
-                    dat = os.path.join(dp, fn)
-        rt = None
-        for line in open(dat):
-            if line.startswith("realtime="):
-                rt = float(line.split("=")[1])
+                    for line in open(os.path.join(dp, fn)):
+                        if line.startswith("realtime="):
+                            return float(line.split("=")[1])
+        raise RuntimeError("no realtime in the output of %s" % exe)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def host_cores():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
+
+
+def cpu_baseline(n, nx, ny):
+    """Times the reference msmain (oracle/_ref, built from the MSModel sources with MKL) on a k-map sample."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "msmain")
+    if not os.path.exists(exe):
+        return None
+    cores = host_cores()
+    try:
+        rt = run_msmain(exe, n, nx, ny, cores)
         evals = nx * ny
         return {"value": evals / rt, "unit": "log-det evals/s", "cores": cores, "kind": "reference",
                 "sample": "msmain -p %d (gcc -O2, MKL sequential per thread): d=3 N=%d unit-density ball seed 1, "
@@ -101,8 +111,22 @@ def cpu_baseline(n, nx, ny):
     except Exception as ex:  # the baseline is reported, never fatal to the GPU measurement
         return {"value": None, "unit": "log-det evals/s", "cores": cores, "kind": "reference",
                 "sample": "failed: %s" % ex}
-    finally:
-        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def dropin_msmain(n, nx, ny):
+    """The drop-in path: the reference msmain relinked against libmsgpu.so (oracle/_ref/msmain_gpu;
+    its matrix functions run on the GPU, concurrent tracelog_lu calls are coalesced), same map."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "msmain_gpu")
+    if not os.path.exists(exe):
+        return None
+    threads = host_cores()
+    try:
+        rt = run_msmain(exe, n, nx, ny, threads)
+        return {"evals_per_s": nx * ny / rt, "threads": threads,
+                "sample": "msmain_gpu -p %d: d=3 N=%d unit-density ball seed 1, hardsphere 0.05, %dx%d k-map "
+                          "= %d log-dets in %.2f s" % (threads, n, nx, ny, nx * ny, rt)}
+    except Exception as ex:
+        return {"evals_per_s": None, "threads": threads, "sample": "failed: %s" % ex}
 
 
 def serving_stream(G, nreq, rate, nmin, nmax, seed=5):
@@ -283,6 +307,7 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the N=200 map (eager vs HIP graph)")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true", help="skip timing msmain relinked against the library")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run this many device copies of 1 GiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -405,6 +430,11 @@ def main():
     breakdown = {c: round(v["ms"] / max(1e-9, sum(x["ms"] for x in prof.values())), 4) for c, v in prof.items()}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, 48, 48)
+    if cpu:
+        progress("cpu baseline done")
+    dropin = None if args.no_dropin else dropin_msmain(n, 48, 48)
+    if dropin:
+        progress("drop-in msmain done")
     line = {
         "metric": "log-det evals/s at N=1000 (mixed vs FP64, same GPU); p50/p99 resonance latency",
         "value": value, "unit": "log-det evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -428,6 +458,7 @@ def main():
         "autotune": tuned,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "dropin_msmain": dropin,
     }
     print(json.dumps(line))
 
