@@ -483,10 +483,17 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
 }
 
 // ------------------------------------------------------------------ in-register triangular solves
-// Four lanes per right-hand side: lane qq of a group holds rows 4m+qq in xv[m].
-// The register file is rotated by one after every 4 rows, so the row being
-// eliminated is always xv[0] (forward) or xv[15] (backward) and every register
-// index is a compile-time constant while the 16-step loop stays rolled.
+// Value of lane q of each quad (DPP quad_perm [q,q,q,q]); q is a constant after unrolling.
+template <class T>
+__device__ __forceinline__ T quad_bcast(T v, int q) {
+    switch (q) {
+        case 0: return dpp_v<0x00>(v);
+        case 1: return dpp_v<0x55>(v);
+        case 2: return dpp_v<0xAA>(v);
+        default: return dpp_v<0xFF>(v);
+    }
+}
+
 // L(row, i) from packed strictly-lower storage (row r at offset r (r - 1) / 2; unit
 // diagonal only).
 template <class T>
@@ -506,16 +513,20 @@ struct PackedUD {
     __device__ __forceinline__ cx<T> operator()(int r, int c) const { return s[(c * (c + 1) >> 1) + r]; }
 };
 
+// Four lanes per right-hand side (a quad): lane qq of a quad holds rows 4m+qq in xv[m],
+// and each solved row reaches the other lanes of its quad by a DPP broadcast.  The
+// register file is rotated by one after every 4 rows, so the row being eliminated is
+// always xv[0] (forward) or xv[15] (backward) and every register index is a
+// compile-time constant while the 16-step loop stays rolled.  Called by whole waves.
 template <class T, bool UNIT = true, class LA>
-__device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], LA sL, int nb, int qq, int lane) {
+__device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], LA sL, int nb, int qq, int /*lane*/) {
 #pragma unroll 1
     for (int ib = 0; ib < 16; ib++) {
 #pragma unroll
         for (int iq = 0; iq < 4; iq++) {
             const int i = 4 * ib + iq;
             if (!UNIT && qq == iq && i < nb) xv[0] = xv[0] / sL(i, i);
-            const int src = (lane & ~3) | iq;
-            const cx<T> xi{__shfl(xv[0].x, src, 64), __shfl(xv[0].y, src, 64)};
+            const cx<T> xi{quad_bcast(xv[0].x, iq), quad_bcast(xv[0].y, iq)};
 #pragma unroll
             for (int m = 0; m < 16; m++) {
                 const int row = 4 * (m + ib) + qq;
@@ -530,15 +541,14 @@ __device__ __forceinline__ void lower_unit_solve(cx<T> (&xv)[16], LA sL, int nb,
 }
 
 template <class T, bool UNIT = false, class LA>
-__device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], LA sL, int qq, int lane) {
+__device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], LA sL, int qq, int /*lane*/) {
 #pragma unroll 1
     for (int ib = 15; ib >= 0; ib--) {
 #pragma unroll
         for (int iq = 3; iq >= 0; iq--) {
             const int i = 4 * ib + iq;
             if (!UNIT && qq == iq) xv[15] = xv[15] / sL(i, i);
-            const int src = (lane & ~3) | iq;
-            const cx<T> xi{__shfl(xv[15].x, src, 64), __shfl(xv[15].y, src, 64)};
+            const cx<T> xi{quad_bcast(xv[15].x, iq), quad_bcast(xv[15].y, iq)};
 #pragma unroll
             for (int m = 0; m < 16; m++) {
                 const int row = 4 * (m - (15 - ib)) + qq;
