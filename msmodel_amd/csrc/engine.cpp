@@ -318,6 +318,20 @@ static Tune tune_for(int np, int prec) {
     if (!best) return Tune{};
     Tune t = *best;
     t.graph = 0;
+    // panel variants name different shapes in each size range (launch_panel): the panel
+    // is that of the nearest entry in np's range, else the range's default
+    auto range = [](int n) { return n <= 1024 ? 0 : n <= 2048 ? 1 : n <= 4096 ? 2 : 3; };
+    const Tune* bp = nullptr;
+    double bpd = 0.0;
+    for (const auto& kv : g_tune) {
+        if (kv.first.second != prec || range(kv.first.first) != range(np)) continue;
+        const double d = std::fabs(std::log((double)kv.first.first / np));
+        if (!bp || d < bpd || (d == bpd && kv.first.first > np)) {
+            bp = &kv.second;
+            bpd = d;
+        }
+    }
+    t.panel = bp ? bp->panel : 0;
     return t;
 }
 template <class T> static int prec_of() { return sizeof(T) == 8 ? 0 : 1; }
