@@ -94,6 +94,8 @@ void       build_ms_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* matrix);
 void       build_normalized_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* matrix);
 void       build_dk_ms_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* matrix);
 void       build_plane_wave(msgpu_medium* med, msgpu_cplx k, double theta, msgpu_cplx* vec);
+/* tracelog_lu: callable from any number of threads at once; calls of the same n that
+ * wait together share one batched factorization and each returns what it returns alone. */
 msgpu_cplx tracelog_lu(int n, msgpu_cplx* matrix);
 msgpu_cplx dk_log_det_ms(msgpu_medium* med, msgpu_cplx k);
 msgpu_cplx traceinv(int n, msgpu_cplx* matrix);
