@@ -312,8 +312,9 @@ def main():
                     help="only run this many device copies of 1 GiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        sys.exit("bench.py: this build runs on one GPU; --gpus %d is not supported (the k points of a map "
-                 "shard across GPUs without exchange, see DESIGN.md)" % args.gpus)
+        sys.exit("bench.py: measures one GPU; --gpus %d is not supported (the k points of a map "
+                 "shard across GPUs without any exchange: each GPU would run this benchmark on its "
+                 "share)" % args.gpus)
 
     if args.copy_probe:
         import torch
