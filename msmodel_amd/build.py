@@ -1,16 +1,21 @@
 """Builds msmodel_amd/libmsgpu.so from csrc/ with hipcc for gfx950 (in-tree).
 
+Every source is compiled to an object under msmodel_amd/_build/ (git-ignored), all
+of them in parallel, and the objects are linked into the shared library.
+
 Usage:  python msmodel_amd/build.py   (run by path: importing the package needs the built library)
 """
 import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libmsgpu.so")
+OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
 
@@ -19,15 +24,31 @@ def sources():
     return sorted(glob.glob(os.path.join(csrc, "*.hip")) + glob.glob(os.path.join(csrc, "*.cpp")))
 
 
-def build(verbose=False):
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + sources() + ["-o", tmp]
-    if verbose:
-        print(" ".join(cmd))
+def _run(cmd):
     res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libmsgpu.so")
+    return res.returncode, res.stdout + res.stderr
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(OBJ, os.path.basename(s) + ".o") for s in srcs]
+    cmds = [[HIPCC] + FLAGS + ["-c", s, "-o", o] for s, o in zip(srcs, objs)]
+    jobs = jobs or min(len(cmds), max(1, os.cpu_count() or 1), 16)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(_run, cmds))
+    for cmd, (rc, out) in zip(cmds, results):
+        if verbose:
+            print(" ".join(cmd))
+        if rc != 0:
+            sys.stderr.write(out)
+            raise RuntimeError("hipcc failed on " + cmd[-3])
+    tmp = LIB + ".tmp"
+    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp]
+    rc, out = _run(link)
+    if rc != 0:
+        sys.stderr.write(out)
+        raise RuntimeError("hipcc failed linking libmsgpu.so")
     os.replace(tmp, LIB)
     return LIB
 
