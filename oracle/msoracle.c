@@ -588,7 +588,10 @@ void orc_kgrid_map(int nx, int ny, double xmin, double xmax, double ymin, double
     }
 }
 
-void orc_kgrid_cut(int ns, orc_cplx zmin, orc_cplx zmax, double dz, orc_cplx* k) {
+void orc_kgrid_cut(int ns, orc_cplx zmin, orc_cplx zmax, orc_cplx* k) {
+    /* the transverse offsets are +-i dz with the complex step dz = (zmax - zmin)/(ns - 1)
+     * (complex_cut_util.c:68): perpendicular to the cut */
+    orc_cplx dz = (zmax - zmin) / (ns - 1);
     for (int l = 0; l < 3 * (ns + 2); l++) {
         int il = l / 3, it = l % 3;
         k[l] = ((ns - il) * zmin + (il - 1) * zmax) / (ns - 1) + I * dz * (it - 1);
@@ -717,4 +720,99 @@ void orc_aberth(int d, int n, const double* pos, int model, const double* p,
     *nz = kept;
     qsort(z, kept, sizeof(orc_cplx), cmp_real);
     free(flag);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Damped Muller root finder on another function of M(k) (root_finder.c:     */
+/* 342-407) with the drivers find_eigenstates_serial_2/_3 (ms_core.c:335-401) */
+/* and the vector filters filter_cvector / uniq_cvector                      */
+/* (complex_vector_util.c:130-177).                                          */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int d, n, method;
+    const double* pos;
+    int model;
+    const double* p;
+    orc_cplx* m;      /* n x n work matrix */
+    orc_cplx* vec;    /* the shared inverse-iteration vector (mineigval) */
+} MullerCtx;
+
+/* f(k) = the eigenvalue reached by inverse iteration (50 steps, 1e-10) from the shared
+ * vector, which keeps the last result (method 1); or 1 / tr M(k)^-1 (method 2). */
+static orc_cplx muller_f(MullerCtx* c, orc_cplx k) {
+    orc_build_matrix(c->d, c->n, c->pos, c->model, c->p, k, c->m);
+    if (c->method == 1) {
+        orc_cplx mu;
+        orc_inverse_iteration(c->n, c->m, c->vec, &mu, 50, 1e-10);
+        return mu;
+    }
+    return 1. / orc_traceinv(c->n, c->m);
+}
+
+static int muller_in_domain(orc_cplx z, double xmin, double xmax, double ymin, double ymax) {
+    return xmin < creal(z) && creal(z) < xmax && ymin < cimag(z) && cimag(z) < ymax;
+}
+
+/* One start: returns 1 on convergence (root in *z), 0 otherwise. */
+static int muller_one(MullerCtx* c, double xmin, double xmax, double ymin, double ymax, orc_cplx* z, int maxit,
+                      double toler) {
+    if (maxit <= 1 || toler <= 0.) return 0;
+    const double h = 1e-4;                        /* relative offsets of the first two points */
+    orc_cplx za = (1. - h * I) * (*z), zb = *z, zc = (1. + h * I) * (*z);
+    orc_cplx fa = muller_f(c, za), fb = muller_f(c, zb), fc = muller_f(c, zc);
+    double absfc = cabs(fc);
+    for (int s = 0; s < maxit; s++) {
+        /* divided differences; the quadratic through the three points, taken at zc */
+        orc_cplx fab = (fa - fb) / (za - zb), fac = (fa - fc) / (za - zc), fbc = (fb - fc) / (zb - zc);
+        orc_cplx d1 = fbc + fac - fab, d2 = (fac - fbc) / (za - zb);
+        orc_cplx sq = csqrt(d1 * d1 - 4. * fc * d2);
+        orc_cplx den = d1 + sq;
+        if (cabs(d1 - sq) > cabs(den)) den = d1 - sq;
+        if (den == 0. || den != den) return 0;
+        orc_cplx dz = 2. * fc / den;
+        double adz = cabs(dz), tol = toler * cabs(zc);
+        if (!muller_in_domain(zc, xmin, xmax, ymin, ymax)) return 0;
+        if (adz < tol || absfc == 0.) { *z = zc; return 1; }
+        /* damping: mu = 1, 1/2, 1/8, 1/64, ... while |f| grows */
+        double mu = 1., div = 2., absfn;
+        orc_cplx zn, fn;
+        do {
+            zn = zc - mu * dz;
+            fn = muller_f(c, zn);
+            absfn = cabs(fn);
+            mu /= div;
+            div *= 2.;
+        } while (absfn > absfc && mu * adz > tol);
+        za = zb; fa = fb;
+        zb = zc; fb = fc;
+        zc = zn; fc = fn; absfc = absfn;
+    }
+    return 0;
+}
+
+int orc_roots_muller(int d, int n, const double* pos, int model, const double* p, double xmin, double xmax,
+                     double ymin, double ymax, int* nz, orc_cplx* z, int maxit, double toler, int method) {
+    MullerCtx c = {d, n, method, pos, model, p, NULL, NULL};
+    c.m = (orc_cplx*)malloc(sizeof(orc_cplx) * (size_t)n * n);
+    c.vec = (orc_cplx*)malloc(sizeof(orc_cplx) * n);
+    for (int i = 0; i < n; i++) c.vec[i] = 1. / sqrt((double)n);      /* constant_unit_cvector */
+    int* conv = (int*)calloc(*nz, sizeof(int));
+    for (int i = 0; i < *nz; i++) {
+        conv[i] = muller_one(&c, xmin, xmax, ymin, ymax, &z[i], maxit, toler);
+        if (method == 1) conv[i] *= (cabs(muller_f(&c, z[i])) < sqrt(toler));
+    }
+    /* keep the converged roots in order, then the first of each group closer than 1e-14 */
+    int size = 0;
+    for (int i = 0; i < *nz; i++)
+        if (conv[i] == 1) { orc_cplx t = z[i]; z[i] = z[size]; z[size] = t; size++; }
+    int u = size > 0 ? 1 : 0;
+    for (int i = 1; i < size; i++) {
+        int uniq = 1;
+        for (int j = 0; j < u; j++)
+            if (cabs(z[i] - z[j]) < 1e-14 * cabs(z[i]) || cabs(z[i] - z[j]) < 1e-14) { uniq = 0; break; }
+        if (uniq) { orc_cplx t = z[i]; z[i] = z[u]; z[u] = t; u++; }
+    }
+    *nz = u;
+    free(c.m); free(c.vec); free(conv);
+    return u;
 }

@@ -23,7 +23,14 @@
  *                      difference of orc_tracelog
  *   orc_gesv           test_ms_matrix.c:229-231 scattering amplitudes, residual
  *   orc_total_wave     direct sum identity at scatterer-free points, fixtures
- *   orc_aberth         N=2 closed-form resonances (mpmath), det M(k*)~0
+ *   orc_aberth         N=2 closed-form resonances (mpmath), det M(k*)~0,
+ *                      root sets of the compiled reference (fixtures, d=2 and d=3)
+ *   orc_roots_muller   root sets of the compiled reference's [eigenstate_list]
+ *                      driver, mineigval and invtraceinv (fixtures)
+ *   orc_inverse_iteration  Mathematica values of test_ms_matrix.c:137-138 (3x3),
+ *                      the compiled reference's eigenvalues and eigenvectors at roots
+ *   orc_kgrid_*        bit-exact vs the compiled reference's get_arg_cmap,
+ *                      get_arg_ccut and init_logcut/get_arg_lcut (fixtures)
  */
 #ifndef MSORACLE_H
 #define MSORACLE_H
@@ -87,7 +94,7 @@ void orc_wavefunction_grid(int source, orc_cplx k, int d, int n, const double* p
 
 /* k-grids (complex_map_util.c:27-31, complex_cut_util.c:32-36, log_cut_util.c:72-101). */
 void orc_kgrid_map(int nx, int ny, double xmin, double xmax, double ymin, double ymax, orc_cplx* k);
-void orc_kgrid_cut(int ns, orc_cplx zmin, orc_cplx zmax, double dz, orc_cplx* k);
+void orc_kgrid_cut(int ns, orc_cplx zmin, orc_cplx zmax, orc_cplx* k);   /* 3 (ns + 2) points */
 void orc_kgrid_logcut(int ns, double rek, double imkmin, double imkmax, orc_cplx* k);
 
 /* Halton (2,3) start points, i = 1..n (domain_util.c:156-185). */
@@ -99,5 +106,13 @@ void orc_halton(int n, double xmin, double xmax, double ymin, double ymax, orc_c
 void orc_aberth(int d, int n, const double* pos, int model, const double* p,
                 double xmin, double xmax, double ymin, double ymax,
                 int* nz, orc_cplx* z, int nexpsup, int maxit, double toler);
+
+/* Roots of det M(k) by the damped Muller method (root_finder.c:342-407) on the smallest
+ * eigenvalue of M(k) by inverse iteration from one vector shared by the starts in turn
+ * (method 1, find_eigenstates_serial_2, ms_core.c:335-367) or on 1/tr M(k)^-1 (method 2,
+ * find_eigenstates_serial_3, ms_core.c:374-401).  The starts z[0..*nz) are run in order;
+ * on exit the converged, distinct (1e-14) roots in that order.  Returns their count. */
+int orc_roots_muller(int d, int n, const double* pos, int model, const double* p, double xmin, double xmax,
+                     double ymin, double ymax, int* nz, orc_cplx* z, int maxit, double toler, int method);
 
 #endif

@@ -81,13 +81,16 @@ def lib():
                                              ctypes.c_double, ctypes.c_double, dp]),
             "orc_kgrid_map": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, dp]),
-            "orc_kgrid_cut": (None, [ctypes.c_int, _C, _C, ctypes.c_double, dp]),
+            "orc_kgrid_cut": (None, [ctypes.c_int, _C, _C, dp]),
             "orc_kgrid_logcut": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]),
             "orc_halton": (None, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, dp]),
             "orc_aberth": (None, [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ip, dp, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_double]),
+            "orc_roots_muller": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, ip, dp,
+                                                ctypes.c_int, ctypes.c_double, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -260,6 +263,45 @@ def kgrid_map(nx, ny, xmin, xmax, ymin, ymax):
     k = np.zeros(nx * ny, dtype=np.complex128)
     lib().orc_kgrid_map(nx, ny, xmin, xmax, ymin, ymax, k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return k
+
+
+def kgrid_cut(ns, zmin, zmax):
+    """The 3 (ns + 2) points of a complex cut, triplet-major (complex_cut_util.c:32-36)."""
+    k = np.zeros(3 * (ns + 2), dtype=np.complex128)
+    lib().orc_kgrid_cut(ns, _c(zmin), _c(zmax), k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return k
+
+
+def kgrid_logcut(ns, rek, imkmin, imkmax):
+    k = np.zeros(ns, dtype=np.complex128)
+    lib().orc_kgrid_logcut(ns, rek, imkmin, imkmax, k.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return k
+
+
+def inverse_iteration(m, v0, maxit, toler):
+    """(converged, normalised vector, eigenvalue) of inverse_iteration (ms_matrix_util.c:279-314)."""
+    a = np.array(m, dtype=np.complex128, order="F", copy=True)
+    v = np.array(v0, dtype=np.complex128, copy=True)
+    lam = np.zeros(1, dtype=np.complex128)
+    conv = lib().orc_inverse_iteration(a.shape[0], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       lam.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), maxit, toler)
+    return conv, v, complex(lam[0])
+
+
+MINEIGVAL, INVTRACEINV = 1, 2
+
+
+def roots_muller(pos, starts, dom, method, model=HARDSPHERE, p=(0.1,), maxit=30, toler=1e-10):
+    """Damped Muller roots of det M(k) from the starts in order (find_eigenstates_serial_2/_3)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n, d = pos.shape
+    pp = _params(p)
+    z = np.array(starts, dtype=np.complex128, copy=True)
+    nz = ctypes.c_int(z.size)
+    lib().orc_roots_muller(d, n, _dp(pos), model, _dp(pp), dom[0], dom[1], dom[2], dom[3], ctypes.byref(nz),
+                           z.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), maxit, toler, method)
+    return z[: nz.value]
 
 
 def halton(n, xmin, xmax, ymin, ymax):

@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(_HERE, "_ref")
 _LIB = None
 _SHIM = None
+_EXT = None
 
 
 class Cplx(ctypes.Structure):
@@ -104,6 +105,54 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def ext():
+    """oracle/_ref/libmsref_ext.so: k-grid generators and the [eigenstate_list] driver."""
+    global _EXT
+    if _EXT is None:
+        lib()
+        E = ctypes.CDLL(os.path.join(REF_DIR, "libmsref_ext.so"))
+        dp = ctypes.POINTER(ctypes.c_double)
+        E.shim_kgrid_map.restype = None
+        E.shim_kgrid_map.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [dp]
+        E.shim_kgrid_cut.restype = None
+        E.shim_kgrid_cut.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4 + [dp]
+        E.shim_kgrid_logcut.restype = None
+        E.shim_kgrid_logcut.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3 + [dp]
+        E.shim_eigenstate_list.restype = ctypes.c_int
+        E.shim_eigenstate_list.argtypes = [ctypes.POINTER(Medium), ctypes.c_int, ctypes.c_int] + \
+            [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_double, dp, dp, dp]
+        _EXT = E
+    return _EXT
+
+
+def kgrid_map(nx, ny, xmin, xmax, ymin, ymax):
+    k = np.zeros(nx * ny, dtype=np.complex128)
+    ext().shim_kgrid_map(nx, ny, xmin, xmax, ymin, ymax, _dp(k))
+    return k
+
+
+def kgrid_cut(ns, zmin, zmax):
+    k = np.zeros(3 * (ns + 2), dtype=np.complex128)
+    zmin, zmax = complex(zmin), complex(zmax)
+    ext().shim_kgrid_cut(ns, zmin.real, zmin.imag, zmax.real, zmax.imag, _dp(k))
+    return k
+
+
+def kgrid_logcut(ns, rek, imkmin, imkmax):
+    k = np.zeros(ns, dtype=np.complex128)
+    ext().shim_kgrid_logcut(ns, rek, imkmin, imkmax, _dp(k))
+    return k
+
+
+def inverse_iteration(m, v0, maxit, toler):
+    """The reference inverse_iteration (ms_matrix_util.c:279-314): (converged, vector, lambda)."""
+    a = np.array(m, dtype=np.complex128, order="F", copy=True)
+    v = np.array(v0, dtype=np.complex128, copy=True)
+    lam = np.zeros(1, dtype=np.complex128)
+    conv = lib().inverse_iteration(a.shape[0], _dp(a), _dp(v), _dp(lam), maxit, toler, 0)
+    return conv, v, complex(lam[0])
+
+
 class RefMedium:
     """A reference Medium built by init_medium; keeps its buffers alive."""
 
@@ -156,6 +205,16 @@ class RefMedium:
         _SHIM.shim_aberth(ctypes.byref(self.med), dom[0], dom[1], dom[2], dom[3], ctypes.byref(nz), _dp(z),
                           nexpsup, maxit, toler)
         return z[: nz.value]
+
+    def eigenstate_list(self, method, ntarget, dom, maxit, toler):
+        """run_eigenstate_list_serial (ms_core.c:406-430) with method 0 determinant, 1 mineigval,
+        2 invtraceinv: the states' roots, eigenvalues mu and eigenvectors (nstate x n)."""
+        kroot = np.zeros(ntarget, dtype=np.complex128)
+        mu = np.zeros(ntarget, dtype=np.complex128)
+        eigv = np.zeros((ntarget, self.n), dtype=np.complex128)
+        ns = ext().shim_eigenstate_list(ctypes.byref(self.med), method, ntarget, dom[0], dom[1], dom[2], dom[3],
+                                        maxit, toler, _dp(kroot), _dp(mu), _dp(eigv))
+        return kroot[:ns], mu[:ns], eigv[:ns]
 
     def wave_grid(self, a, k, nx, ny, xmin, xmax, ymin, ymax, source=0):
         lib()

@@ -27,7 +27,7 @@ CORE_OBJ := $(patsubst %,$(OBJ)/%.o,$(CORE))
 ALL_SRC  := $(wildcard $(SRC)/*.c)
 ALL_OBJ  := $(patsubst $(SRC)/%.c,$(OBJ)/%.o,$(ALL_SRC))
 
-all: $(OUT)/libmsref.so $(OUT)/libmsref_shim.so $(OUT)/msmain
+all: $(OUT)/libmsref.so $(OUT)/libmsref_shim.so $(OUT)/libmsref_ext.so $(OUT)/msmain
 
 # msmain_gpu needs msmodel_amd/libmsgpu.so (python msmodel_amd/build.py); not part of `all`.
 gpu: $(OUT)/msmain_gpu
@@ -46,6 +46,11 @@ $(OUT)/libmsref.so: $(CORE_OBJ)
 
 $(OUT)/libmsref_shim.so: oracle/ref_shim.c $(OUT)/libmsref.so
 	$(CC) $(CFLAGS) -shared $< -L$(OUT) -lmsref -Wl,-rpath,'$$ORIGIN' $(LDLIBS) -o $@
+
+# Every core object (grid generators, eigenstate drivers) behind the plain-C entry
+# points of oracle/ref_ext_shim.c.
+$(OUT)/libmsref_ext.so: oracle/ref_ext_shim.c $(ALL_OBJ)
+	$(CC) $(CFLAGS) -shared $< $(ALL_OBJ) $(LDLIBS) -o $@
 
 $(OUT)/msmain: $(ALL_OBJ) $(OBJ)/ms_main.o
 	$(CC) -fopenmp $^ $(LDLIBS) -o $@

@@ -185,5 +185,50 @@ put("ab_d3_starts", starts)
 put("ab_d3_dom", np.array(dom))
 put("ab_d3_roots", m.aberth(starts, dom, nexpsup=4, maxit=60, toler=1e-12))
 
+# ----------------------------------------------------------------- k-point generators (bit-exact)
+# get_arg_cmap (complex_map_util.c:27-31), get_arg_ccut (complex_cut_util.c:32-36),
+# init_logcut + get_arg_lcut (log_cut_util.c:72-101)
+grid_map = [(13, 7, 1.0, 20.0, -1.0, 0.0), (64, 64, 5.0, 30.0, -2.0, 0.0), (5, 3, -2.5, 7.25, -3.0, 0.5)]
+put("grid_map_spec", np.array(grid_map, dtype=np.float64))
+for i, g in enumerate(grid_map):
+    put("grid_map_%d" % i, R.kgrid_map(*g))
+grid_cut = [(4, 1 - 0.5j, 5 - 0.1j), (7, 2 + 0j, 12 - 1j), (20, 3 - 0.01j, 3 - 2j), (33, 0.5 - 1j, 40 - 0.2j)]
+put("grid_cut_ns", np.array([g[0] for g in grid_cut], dtype=np.int64))
+put("grid_cut_ends", np.array([[g[1], g[2]] for g in grid_cut], dtype=np.complex128))
+for i, g in enumerate(grid_cut):
+    put("grid_cut_%d" % i, R.kgrid_cut(*g))
+grid_log = [(37, 12.5, -1e-4, -2.0), (10, 5.0, -1.0, -1e-3), (2, 30.0, -0.5, -0.25)]
+put("grid_log_spec", np.array(grid_log, dtype=np.float64))
+for i, g in enumerate(grid_log):
+    put("grid_log_%d" % i, R.kgrid_logcut(g[0], *g[1:]))
+
+# ----------------------------------------------------------------- inverse iteration (3x3 test matrix)
+conv, v, lam = R.inverse_iteration(A3, np.ones(3) / np.sqrt(3), 30, 1e-12)
+put("a3_ivit_vec", v)
+put("a3_ivit_lambda", lam)
+
+# ----------------------------------------------------------------- [eigenstate_list] driver, all methods
+# run_eigenstate_list_serial (ms_core.c:406-430): Halton starts sorted by -Im k, the root
+# finder of the method (0 determinant/Aberth, 1 mineigval/Muller, 2 invtraceinv/Muller),
+# then compute_eigenstates: per state the root, mu and the eigenvector (50 inverse
+# iterations, 1e-10, from the constant unit vector).
+eig_cases = [  # tag, d, n, seed, alpha, dom, ntarget, maxit, toler, method
+    ("det_d2", 2, 20, 1, 0.4, (1.0, 10.0, -1.0, 0.0), 12, 60, 1e-12, 0),
+    ("mev_d2", 2, 20, 1, 0.4, (1.0, 10.0, -1.0, 0.0), 12, 30, 1e-12, 1),
+    ("iti_d2", 2, 20, 1, 0.4, (1.0, 10.0, -1.0, 0.0), 12, 30, 1e-12, 2),
+    ("mev_d3", 3, 30, 2, 0.1, (2.0, 12.0, -1.5, 0.0), 10, 30, 1e-12, 1),
+    ("iti_d3", 3, 30, 2, 0.1, (2.0, 12.0, -1.5, 0.0), 10, 30, 1e-12, 2),
+    ("mev_d2n60", 2, 60, 3, 0.2, (2.0, 9.0, -0.8, 0.0), 16, 30, 1e-12, 1),
+    ("iti_d2n60", 2, 60, 3, 0.2, (2.0, 9.0, -0.8, 0.0), 16, 30, 1e-12, 2),
+]
+for tag, d, n, seed, alpha, dom, nt, maxit, toler, method in eig_cases:
+    m = medium(d, n, seed, p=(alpha,))
+    kr, mu, ev = m.eigenstate_list(method, nt, dom, maxit, toler)
+    put("eig_%s_spec" % tag, np.array([d, n, seed, alpha, dom[0], dom[1], dom[2], dom[3], nt, maxit, toler, method]))
+    put("eig_%s_pos" % tag, m.pos)
+    put("eig_%s_kroot" % tag, kr)
+    put("eig_%s_mu" % tag, mu)
+    put("eig_%s_eigv" % tag, ev)
+
 np.savez_compressed(OUT, **out)
 print("wrote", OUT, os.path.getsize(OUT), "bytes,", len(out), "arrays")

@@ -51,7 +51,7 @@ def test_map_n200(G):
     for l in np.random.default_rng(1).choice(ks.size, 48, replace=False):
         want = O.tracelog(O.build_matrix(pos, ks[l], HS, (0.1,)))
         assert abs(out[l].real - want.real) <= 1e-10 * max(1.0, abs(want.real)), l
-        assert abs(wrapped(200 * (out[l].imag - want.imag))) <= 1e-8, l
+        assert abs(wrapped(200 * (out[l].imag - want.imag))) <= 1e-10, l
 
 
 def test_solve_and_psi_n1000(G):

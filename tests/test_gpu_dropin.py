@@ -2,14 +2,20 @@
 wavefunction calls resolve to libmsgpu.so (oracle/_ref/msmain_gpu, made by
 `make -f oracle/ref.mk gpu`), against the unmodified msmain (MKL) on the same
 configuration files.  Both programs run the reference's task drivers
-(run_tracelog_omp, run_wave_function_omp, find_eigenstates_serial_1) with two
-OpenMP threads, so the library is also called from concurrent host threads.
+(run_tracelog_omp over maps, cuts and log cuts, run_wave_function_omp, and the
+[eigenstate_list] driver with each of its three root finders:
+find_eigenstates_serial_1/_2/_3 then compute_eigenstates) with two OpenMP
+threads, so the library is also called from concurrent host threads.
 
 Compared outputs (the .dat file msmain writes, complex_data_util.c:21-28,
-wave_function_util.c:228-245, eigenstate_list_util.c:296-317):
-  [kplane] data      tracelog map: Re to 1e-10 relative, N*Im modulo 2 pi to 1e-8
+wave_function_util.c:228-245, eigenstate_list_util.c:296-317, eigenstate_util.c:202-213):
+  [kplane] [kcut] [logkcut] data   tracelog: Re to 1e-10 relative, N*Im modulo 2 pi to 1e-10
   [wavefunction]     Re psi and |psi|^2 on the grid: 1e-9 of the grid maximum
-  [eigenstate_list]  the root sets: every root within 1e-9 of one root of the other
+  [eigenstate_list]  the root sets both ways (1e-10 |k|); per state mu to 1e-12 and the
+                     phase-normalised eigenvector to 1e-10
+
+The two executables are built in the container that holds the MSModel sources and
+travel with the tree; the test fails (it does not skip) when they are missing.
 """
 import os
 import subprocess
@@ -23,9 +29,7 @@ REF = os.path.join(ROOT, "oracle", "_ref")
 CPU_EXE = os.path.join(REF, "msmain")
 GPU_EXE = os.path.join(REF, "msmain_gpu")
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not (os.path.exists(CPU_EXE) and os.path.exists(GPU_EXE)),
-                                 reason="oracle/_ref/msmain and msmain_gpu are built from the MSModel sources")]
+pytestmark = pytest.mark.gpu
 
 CONF_D2 = """[medium]
 dimension=2
@@ -55,6 +59,21 @@ yrange=-4:4
 color1=temperature div 5
 color2=temperature div 5
 
+[kcut]
+nseed=1
+kmin=2-0.8i
+kmax=7-0.1i
+sample=9
+title=dropincut
+
+[logkcut]
+nseed=1
+rek=4.5
+imkmin=-0.001
+imkmax=-1.5
+sample=11
+title=dropinlog
+
 [eigenstate_list]
 seed=1
 ntarget=12
@@ -63,6 +82,26 @@ yrange=-1:0
 method=determinant
 maxit=60
 toler=1e-10
+verb=0
+
+[eigenstate_list]
+seed=1
+ntarget=12
+xrange=2:6
+yrange=-1:0
+method=mineigval
+maxit=30
+toler=1e-12
+verb=0
+
+[eigenstate_list]
+seed=1
+ntarget=12
+xrange=2:6
+yrange=-1:0
+method=invtraceinv
+maxit=30
+toler=1e-12
 verb=0
 """
 
@@ -97,6 +136,7 @@ color2=temperature div 5
 
 
 def run(exe, conf, tmp):
+    assert os.path.exists(exe), "%s is missing: build it with `make -f oracle/ref.mk all gpu`" % exe
     os.makedirs(tmp, exist_ok=True)
     path = os.path.join(tmp, "t.conf")
     with open(path, "w") as f:
@@ -111,11 +151,13 @@ def run(exe, conf, tmp):
 
 
 def parse_dat(path):
+    """{section name: [one dict per occurrence, in file order]}"""
     sections, cur = {}, None
     for line in open(path):
         line = line.strip()
         if line.startswith("[") and line.endswith("]"):
-            cur = sections.setdefault(line[1:-1], {})
+            cur = {}
+            sections.setdefault(line[1:-1], []).append(cur)
         elif "=" in line and cur is not None:
             key, val = line.split("=", 1)
             cur[key] = val
@@ -137,34 +179,56 @@ def outputs(request, tmp_path_factory):
     return run(CPU_EXE, conf, str(base / "cpu")), run(GPU_EXE, conf, str(base / "gpu"))
 
 
-def test_kplane_map(outputs):
+@pytest.mark.parametrize("section", ["kplane", "kcut", "logkcut"])
+def test_tracelog_tasks(outputs, section):
     cpu, gpu = outputs
-    n = int(cpu["medium"]["natom"])
-    a, b = cvec(cpu["kplane"]["data"]), cvec(gpu["kplane"]["data"])
-    assert a.shape == b.shape and a.size > 0
-    assert np.abs(a.real - b.real).max() <= 1e-10 * max(1.0, np.abs(a.real).max())
-    assert np.abs(wrapped(n * (a.imag - b.imag))).max() <= 1e-8
+    if section not in cpu:
+        pytest.skip("no %s section in this configuration" % section)
+    n = int(cpu["medium"][0]["natom"])
+    for sc, sg in zip(cpu[section], gpu[section]):
+        a, b = cvec(sc["data"]), cvec(sg["data"])
+        assert a.shape == b.shape and a.size > 0
+        ere = np.abs(a.real - b.real).max() / max(1.0, np.abs(a.real).max())
+        earg = np.abs(wrapped(n * (a.imag - b.imag))).max()
+        print("[%s] %d cells: max rel |d Re| %.2e, max |d arg det| %.2e" % (section, a.size, ere, earg))
+        assert ere <= 1e-10
+        assert earg <= 1e-10
 
 
 def test_wavefunction(outputs):
     cpu, gpu = outputs
     for key in ("function", "density"):
-        a, b = rvec(cpu["wavefunction"][key]), rvec(gpu["wavefunction"][key])
+        a, b = rvec(cpu["wavefunction"][0][key]), rvec(gpu["wavefunction"][0][key])
         assert a.shape == b.shape and a.size > 0
         assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max(), key
 
 
-def test_eigenstate_roots(outputs):
+def test_eigenstate_lists(outputs):
     cpu, gpu = outputs
     if "eigenstate_list" not in cpu:
         pytest.skip("no eigenstate section in this configuration")
+    assert len(cpu["eigenstate_list"]) == len(gpu["eigenstate_list"])
+    for sc, sg in zip(cpu["eigenstate_list"], gpu["eigenstate_list"]):
+        assert sc["method"] == sg["method"]
 
-    def roots(sec):
-        return np.array([complex(sec["st%d_kroot" % i].replace("i", "j")) for i in range(int(sec["nstate"]))])
+        def states(sec):
+            ns = int(sec["nstate"])
+            k = np.array([complex(sec["st%d_kroot" % i].replace("i", "j")) for i in range(ns)])
+            mu = np.array([complex(sec["st%d_mu" % i].replace("i", "j")) for i in range(ns)])
+            v = [cvec(sec["st%d_eigv" % i]) for i in range(ns)]
+            return k, mu, v
 
-    a, b = roots(cpu["eigenstate_list"]), roots(gpu["eigenstate_list"])
-    assert a.size > 0 and a.size == b.size
-    for z in a:
-        assert np.abs(b - z).min() <= 1e-9 * max(1.0, abs(z)), z
-    for z in b:
-        assert np.abs(a - z).min() <= 1e-9 * max(1.0, abs(z)), z
+        (ka, mua, va), (kb, mub, vb) = states(sc), states(sg)
+        assert ka.size > 0 and ka.size == kb.size, (sc["method"], ka, kb)
+        for z in ka:
+            assert np.abs(kb - z).min() <= 1e-10 * abs(z), (sc["method"], z)
+        for z in kb:
+            assert np.abs(ka - z).min() <= 1e-10 * abs(z), (sc["method"], z)
+        # the states in the same order (sorted by -Im k): eigenvalue and eigenvector
+        dmu = max(abs(x - y) for x, y in zip(mua, mub))
+        dv = max(np.abs(x - y).max() for x, y in zip(va, vb))
+        print("[eigenstate_list] %s: %d states, max |d k| %.2e, max |d mu| %.2e, max |d v| %.2e"
+              % (sc["method"], ka.size, np.abs(ka - kb).max(), dmu, dv))
+        assert np.abs(ka - kb).max() <= 1e-10 * np.abs(ka).max()
+        assert dmu <= 1e-12
+        assert dv <= 1e-10

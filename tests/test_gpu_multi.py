@@ -78,7 +78,7 @@ def test_tracelog_multi_vs_oracle(G, media):
         want = O.tracelog(O.build_matrix(pos, ks[i], model, p))
         n = pos.shape[0]
         assert abs(got[i].real - want.real) <= 1e-10 * max(1.0, abs(want.real)), i
-        assert abs(wrapped(n * (got[i].imag - want.imag))) <= 1e-8, i
+        assert abs(wrapped(n * (got[i].imag - want.imag))) <= 1e-10, i
 
 
 def test_dk_multi_equals_single_and_oracle(G, media):
@@ -149,6 +149,8 @@ def test_resonances_multi_equals_single(G, ref):
             want, _ = G.resonances(m, s, dm, 4, 60, 1e-12, mode)
             assert len(r) == len(want)
             assert np.allclose(np.sort_complex(r), np.sort_complex(want), rtol=1e-12, atol=0)
-    # medium a's roots are the reference's (root sets, as assert_cvector_subset)
-    found = [np.min(np.abs(roots[0] - w)) < 1e-10 * abs(w) for w in ref["ab_d2_roots"]]
-    assert sum(found) >= max(1, len(ref["ab_d2_roots"]) // 2)
+        # media a and b give the reference's root sets, in both directions
+        for r, want in ((roots[0], ref["ab_d2_roots"]), (roots[1], ref["ab_d3_roots"])):
+            assert len(r) == len(want), (mode, r, want)
+            for w in want:
+                assert np.min(np.abs(r - w)) < 1e-10 * abs(w), (mode, w)

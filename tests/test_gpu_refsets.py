@@ -1,0 +1,94 @@
+"""Root sets and eigenstates of the HIP path against those of the compiled reference
+(tests/golden/ref_fixtures.npz, written by the reference's own drivers):
+
+  Aberth on det M    msgpu_resonances vs solve_aberth + dk_log_det_ms (root_finder.c:522-577,
+                     ms_core.c:320-328), d=2 N=20 and d=3 N=30, FP64 and MIXED
+  damped Muller      msgpu_roots_muller vs find_eigenstates_serial_2/_3 (ms_core.c:335-401,
+                     root_finder.c:342-407): smallest eigenvalue and 1/tr M^-1, three media
+  eigenstates        msgpu_eigenstates vs compute_eigenstates (ms_core.c:296-313): mu and the
+                     phase-normalised eigenvector that save_eigenstate writes
+
+Root sets are compared in both directions (assert_cvector_subset, src/test/assertion.c:136-162):
+every reference root has a GPU root within 1e-10 |k| and every GPU root a reference root.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+HS = O.HARDSPHERE
+
+
+@pytest.fixture(scope="module")
+def G():
+    import msmodel_amd as G
+    G.init()
+    return G
+
+
+def medium(G, pos, p):
+    pos = np.asarray(pos)
+    n, d = pos.shape
+    m = G.Medium(d, n, HS, p)
+    m.set_pos(pos)
+    return m
+
+
+def same_root_sets(got, want, rtol=1e-10):
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.size == want.size and want.size > 0, (np.sort_complex(got), np.sort_complex(want))
+    err = max(max(np.min(np.abs(got - w)) / abs(w) for w in want), max(np.min(np.abs(want - g)) / abs(g) for g in got))
+    print("%d roots, max relative distance to the reference set %.2e" % (want.size, err))
+    assert err < rtol, err
+
+
+def eig_case(ref, tag):
+    spec = ref["eig_%s_spec" % tag]
+    return dict(pos=ref["eig_%s_pos" % tag], alpha=spec[3], dom=tuple(spec[4:8]), nt=int(spec[8]),
+                maxit=int(spec[9]), toler=spec[10], method=int(spec[11]), kroot=ref["eig_%s_kroot" % tag],
+                mu=ref["eig_%s_mu" % tag], eigv=ref["eig_%s_eigv" % tag])
+
+
+def halton_starts(nt, dom):
+    """Starts of the [eigenstate_list] driver: Halton (2,3) points sorted by -Im k, stable
+    (run_eigenstate_list_serial, ms_core.c:413-415)."""
+    z = O.halton(nt, *dom)
+    return z[np.argsort(-z.imag, kind="stable")]
+
+
+@pytest.mark.parametrize("mode", ["FP64", "MIXED"])
+@pytest.mark.parametrize("case", ["d2", "d3"])
+def test_aberth_root_sets(G, ref, case, mode):
+    pos = O.fill_medium(2, 20, 1) if case == "d2" else ref["ab_d3_pos"]
+    m = medium(G, pos, (0.4,) if case == "d2" else (0.1,))
+    roots, sweeps = G.resonances(m, ref["ab_%s_starts" % case], tuple(ref["ab_%s_dom" % case]), 4, 60, 1e-12,
+                                 getattr(G, mode))
+    assert sweeps > 0
+    same_root_sets(roots, ref["ab_%s_roots" % case])
+
+
+@pytest.mark.parametrize("tag", ["mev_d2", "iti_d2", "mev_d3", "iti_d3", "mev_d2n60", "iti_d2n60"])
+def test_muller_root_sets(G, ref, tag):
+    c = eig_case(ref, tag)
+    m = medium(G, c["pos"], (c["alpha"],))
+    method = G.ROOT_MINEIGVAL if c["method"] == 1 else G.ROOT_INVTRACEINV
+    roots, rounds = G.roots_muller(m, halton_starts(c["nt"], c["dom"]), c["dom"], c["maxit"], c["toler"], method)
+    assert rounds > 0
+    same_root_sets(roots, c["kroot"])
+
+
+@pytest.mark.parametrize("tag", ["det_d2", "mev_d3", "iti_d2n60"])
+def test_eigenstates_vs_reference(G, ref, tag):
+    c = eig_case(ref, tag)
+    m = medium(G, c["pos"], (c["alpha"],))
+    vecs, mu, conv, suspect = G.eigenstates(m, c["kroot"], 50, 1e-10)
+    dmu = np.abs(mu - c["mu"]).max()
+    dv = np.abs(vecs - c["eigv"]).max()
+    print("%d states: max |d mu| %.2e, max |d v| %.2e" % (len(mu), dmu, dv))
+    assert dmu < 1e-12
+    assert dv < 1e-10
+    # check_eigenstate_list flags states within 1e-6 |k| of an earlier root or with |mu| > 1e-6
+    for i, k in enumerate(c["kroot"]):
+        dup = any(abs(k - c["kroot"][j]) < 1e-6 * abs(k) for j in range(i))
+        assert bool(suspect[i]) == (dup or abs(c["mu"][i]) > 1e-6), i

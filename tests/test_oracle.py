@@ -240,10 +240,22 @@ def test_solve_and_wavefunction_fixtures(ref, tag, model, p, k, grid, source):
     assert np.abs(psi[fin] - want[fin]).max() < 1e-10 * np.abs(want[fin]).max()
 
 
-def test_traceinv_and_inverse_iteration():
+def test_traceinv_and_inverse_iteration(ref):
     a3 = np.array([-1.997 + 0.45j, 1.875 - 1.47j, -1.641 + 0.204j, -1.74 - 1.613j, -1.197 - 1.071j,
                    1.883 - 1.765j, 1.958 + 0.195j, 1.278 + 1.105j, -1.08 + 1.842j]).reshape(3, 3, order="F")
     assert abs(O.traceinv(a3) - (-2.5885257744954697 - 0.8796395991818305j)) < 1e-13
+    # inverse iteration: the Mathematica eigenpair of test_ms_matrix.c:137-138 (30 steps, 1e-12,
+    # from the constant unit vector), and the compiled reference's own result
+    conv, v, lam = O.inverse_iteration(a3, np.ones(3) / np.sqrt(3), 30, 1e-12)
+    assert conv == 1
+    assert abs(lam - (-0.38598154234503224 + 0.15816377074842095j)) < 1e-12
+    assert np.abs(v - np.array([0.16644079736120876, 0.4293668879243407 + 0.46904982189636174j,
+                                0.2043459505806509 + 0.7253802682650067j])).max() < 1e-10
+    assert abs(lam - ref["a3_ivit_lambda"]) < 1e-14
+    assert np.abs(v - ref["a3_ivit_vec"]).max() < 1e-14
+    # a wrong eigenvalue would fail: the other two eigenvalues of a3 are far from lam
+    ev = np.linalg.eigvals(a3)
+    assert np.min(np.abs(ev - lam)) < 1e-12 and np.sort(np.abs(ev))[1] > 2 * abs(lam)
 
 
 # --------------------------------------------------------------------- grids and roots
@@ -252,13 +264,88 @@ def test_kgrid_map_order():
     assert k[0] == 1 + 0j and k[3] == 4 + 0j and k[4] == 1 - 0.5j and k[11] == 4 - 1j
 
 
+def test_kgrids_bit_exact_vs_reference(ref):
+    """get_arg_cmap, get_arg_ccut and the log-cut mesh of the compiled reference, bit for bit."""
+    for i, g in enumerate(ref["grid_map_spec"]):
+        got = O.kgrid_map(int(g[0]), int(g[1]), *g[2:])
+        assert np.array_equal(got, ref["grid_map_%d" % i]), i
+    for i, (ns, ends) in enumerate(zip(ref["grid_cut_ns"], ref["grid_cut_ends"])):
+        got = O.kgrid_cut(int(ns), ends[0], ends[1])
+        assert got.size == 3 * (ns + 2)
+        assert np.array_equal(got, ref["grid_cut_%d" % i]), i
+    for i, g in enumerate(ref["grid_log_spec"]):
+        got = O.kgrid_logcut(int(g[0]), *g[1:])
+        assert np.array_equal(got, ref["grid_log_%d" % i]), i
+
+
+def test_kgrid_cut_geometry():
+    """The triplets straddle the cut perpendicularly: the middle point of triplet il is on the
+    segment (il - 1 steps from zmin), the side points are +-i dz away (dz the complex step)."""
+    zmin, zmax, ns = 1 - 0.5j, 5 - 0.1j, 6
+    k = O.kgrid_cut(ns, zmin, zmax).reshape(ns + 2, 3)
+    dz = (zmax - zmin) / (ns - 1)
+    assert np.allclose(k[:, 1], zmin + dz * (np.arange(ns + 2) - 1), rtol=0, atol=1e-14)
+    assert np.allclose(k[:, 2] - k[:, 1], 1j * dz, rtol=0, atol=1e-14)
+    assert np.allclose(k[:, 0] - k[:, 1], -1j * dz, rtol=0, atol=1e-14)
+
+
+def test_kgrid_logcut_geometry():
+    k = O.kgrid_logcut(9, 7.5, -1e-3, -10.0)
+    assert np.all(k.real == 7.5)
+    assert abs(k[0].imag + 1e-3) < 1e-18 and abs(k[-1].imag + 10.0) < 1e-12
+    assert np.allclose(np.diff(np.log(-k.imag)), np.log(1e4) / 8, rtol=1e-12)
+
+
+def assert_same_root_sets(got, want, rtol):
+    """Both directions of assert_cvector_subset (src/test/assertion.c:136-162)."""
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.size == want.size and want.size > 0, (got, want)
+    err = max(max(np.min(np.abs(got - w)) / abs(w) for w in want), max(np.min(np.abs(want - g)) / abs(g) for g in got))
+    print("root sets: %d roots, max relative distance %.2e" % (want.size, err))
+    assert err < rtol, err
+
+
 def test_aberth_fixtures(ref):
     pos = O.fill_medium(2, 20, 1)
     roots = O.aberth(pos, ref["ab_d2_starts"], tuple(ref["ab_d2_dom"]), HS, (0.4,), 4, 60, 1e-12)
-    want = ref["ab_d2_roots"]
-    assert len(roots) == len(want) and len(want) > 0
-    for w in want:
-        assert np.min(np.abs(roots - w)) < 1e-9 * abs(w)
+    assert_same_root_sets(roots, ref["ab_d2_roots"], 1e-10)
+    roots = O.aberth(ref["ab_d3_pos"], ref["ab_d3_starts"], tuple(ref["ab_d3_dom"]), HS, (0.1,), 4, 60, 1e-12)
+    assert_same_root_sets(roots, ref["ab_d3_roots"], 1e-10)
+
+
+def eig_case(ref, tag):
+    spec = ref["eig_%s_spec" % tag]
+    d, n, seed = int(spec[0]), int(spec[1]), int(spec[2])
+    return dict(pos=ref["eig_%s_pos" % tag], alpha=spec[3], dom=tuple(spec[4:8]), nt=int(spec[8]),
+                maxit=int(spec[9]), toler=spec[10], method=int(spec[11]), kroot=ref["eig_%s_kroot" % tag],
+                mu=ref["eig_%s_mu" % tag], eigv=ref["eig_%s_eigv" % tag])
+
+
+def halton_starts(nt, dom):
+    """The [eigenstate_list] starts: Halton points sorted by -Im k, stable (ms_core.c:413-415)."""
+    z = O.halton(nt, *dom)
+    return z[np.argsort(-z.imag, kind="stable")]
+
+
+@pytest.mark.parametrize("tag", ["mev_d2", "iti_d2", "mev_d3", "iti_d3", "mev_d2n60", "iti_d2n60"])
+def test_muller_root_sets_vs_reference(ref, tag):
+    c = eig_case(ref, tag)
+    roots = O.roots_muller(c["pos"], halton_starts(c["nt"], c["dom"]), c["dom"], c["method"], HS, (c["alpha"],),
+                           c["maxit"], c["toler"])
+    assert_same_root_sets(roots, c["kroot"], 1e-10)
+
+
+@pytest.mark.parametrize("tag", ["det_d2", "mev_d3", "iti_d2n60"])
+def test_eigenvectors_vs_reference(ref, tag):
+    """compute_eigenstates (ms_core.c:296-313): at each root, 50 inverse iterations from the
+    constant unit vector; mu and the phase-normalised eigenvector as save_eigenstate writes them."""
+    c = eig_case(ref, tag)
+    n = c["pos"].shape[0]
+    for k, mu, v in zip(c["kroot"], c["mu"], c["eigv"]):
+        m = O.build_matrix(c["pos"], k, HS, (c["alpha"],))
+        conv, vo, lam = O.inverse_iteration(m, np.ones(n) / np.sqrt(n), 50, 1e-10)
+        assert abs(lam - mu) < 1e-12, (k, lam, mu)
+        assert np.abs(vo - v).max() < 1e-10, (k, np.abs(vo - v).max())
 
 
 def test_aberth_n2_closed_form():
