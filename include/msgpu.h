@@ -159,7 +159,10 @@ int msgpu_dk_log_det_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks
 /* Foldy-Lax amplitudes a = M(k)^-1 phi with phi the incident wave at the scatterers
  * (ms_core.c:616-620).  mode MSGPU_FP64: complex128 LU; MSGPU_MIXED: complex64 LU
  * with FP64 iterative refinement until ||M a - phi|| <= tol ||phi|| (tol<=0: 1e-12).
- * Returns the number of refinement steps (>=0) or <0 on error. */
+ * When the refinement stalls (the residual fails to halve in two consecutive steps,
+ * as near a resonance where cond(M) eps32 >~ 1) or has not met tol after 30 steps,
+ * the amplitudes are recomputed with the complex128 LU, which counts as one more step.
+ * Returns the number of steps (>=0) or <0 on error. */
 int msgpu_solve(const msgpu_medium* med, const msgpu_iwave* iwave, msgpu_cplx* a, int mode, double tol);
 /* Roots of det M(k) in the open domain (xmin, xmax) x (ymin, ymax) by the damped Muller
  * method (find_root_muller, root_finder.c:342-407) on f(k) = smallest eigenvalue of M(k)

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <condition_variable>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -96,6 +97,13 @@ static Pool& pinned_pool() {
     return *p;
 }
 
+// When a buffer is released while an exception unwinds the stack (a HIP call failed
+// after kernels were queued), work still in flight may write it: the device is drained
+// before the buffer goes back to the pool that other threads draw from.
+static void drain_if_unwinding() {
+    if (std::uncaught_exceptions() > 0) (void)hipDeviceSynchronize();
+}
+
 template <class T> struct DBuf {
     T* p = nullptr;
     size_t cls = 0;
@@ -108,7 +116,10 @@ template <class T> struct DBuf {
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() {
-        if (p) pool().put(p, cls);
+        if (p) {
+            drain_if_unwinding();
+            pool().put(p, cls);
+        }
     }
 };
 
@@ -1093,15 +1104,35 @@ int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>*
             HIPCHK(hipMemcpyAsync(hphi.data(), phi.p, sizeof(cx<double>) * np, hipMemcpyDeviceToHost, L.st));
             L.sync();
             const double nphi = host_norm(hphi, n);
-            for (steps = 0; steps < 60; steps++) {
+            // Refinement converges when cond(M) eps32 < 1; near a resonance it stalls or
+            // diverges.  It stops when the residual meets tol, or when it has failed to
+            // halve in two consecutive steps, or after kMaxSteps steps; in the last two
+            // cases the amplitudes are recomputed with the complex128 LU, so the result
+            // always meets the complex128 solve's accuracy.
+            constexpr int kMaxSteps = 30;
+            double prev = INFINITY;
+            int slow = 0;
+            bool met = false;
+            for (steps = 0; steps < kMaxSteps; steps++) {
                 launch_residual(m64.p, x.p, phi.p, np, r.p, L.st);
                 HIPCHK(hipMemcpyAsync(hr.data(), r.p, sizeof(cx<double>) * np, hipMemcpyDeviceToHost, L.st));
                 L.sync();
-                if (host_norm(hr, n) <= tol * nphi) break;
+                const double rn = host_norm(hr, n);
+                if (rn <= tol * nphi) { met = true; break; }
+                slow = (rn > 0.5 * prev) ? slow + 1 : 0;
+                if (slow >= 2 || !std::isfinite(rn)) break;
+                prev = rn;
                 launch_convert<double, float>(r.p, v32.p, np, L.st);
                 lu_solve_rm<float>(m32.p, v32.p, np, 1, 1, np, L.st);
                 launch_scatter_rows<float>(d32.p, v32.p, perm.p, np, 1, np, np, 1, L.st);
                 launch_axpy<float>(x.p, d32.p, np, L.st);
+            }
+            if (!met) {
+                factor<double>(m64.p, np, 1, ipiv.p, info.p, pmap.p, true, true, L.st);
+                launch_build_perm(pmap.p, np, NB, perm.p, 1, L.st);
+                lu_solve_rm<double>(m64.p, phi.p, np, 1, 1, np, L.st);
+                launch_scatter_rows<double>(x.p, phi.p, perm.p, np, 1, np, np, 1, L.st);
+                steps++;         // the complex128 solve counts as one more step
             }
         }
         int hinfo = 0;
@@ -1299,7 +1330,10 @@ struct HostStage {
     explicit HostStage(size_t count) { p = (cx<double>*)pinned_pool().get(sizeof(cx<double>) * count, cls); }
     HostStage(const HostStage&) = delete;
     HostStage& operator=(const HostStage&) = delete;
-    ~HostStage() { pinned_pool().put(p, cls); }
+    ~HostStage() {
+        drain_if_unwinding();
+        pinned_pool().put(p, cls);
+    }
 };
 // stage <- host src (ld lds), rows x cols, packed (ld rows)
 static void pack_block(const HostStage& hs, const cx<double>* src, int lds, int rows, int cols) {
@@ -1499,8 +1533,11 @@ int tracelog_host_coalesced(int n, cx<double>* a, int lda, int* ipiv, cx<double>
             g_tl_active++;
             std::vector<TlReq*> batch;
             const int bn = g_tl_queue.front()->n;
+            // at most 256 matrices, and no more than the workspace budget holds (one at least)
+            const size_t cap = std::max<size_t>(
+                1, std::min<size_t>(256, batch_budget() / (sizeof(cx<double>) * (size_t)padded(bn) * padded(bn))));
             for (size_t i = 0; i < g_tl_queue.size();) {
-                if (g_tl_queue[i]->n == bn && batch.size() < 256) {
+                if (g_tl_queue[i]->n == bn && batch.size() < cap) {
                     batch.push_back(g_tl_queue[i]);
                     g_tl_queue.erase(g_tl_queue.begin() + (long)i);
                 } else {

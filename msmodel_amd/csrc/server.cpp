@@ -5,8 +5,11 @@
 //   bytes they hold (16 N^2 per complex128 matrix, 8 N^2 per complex64 matrix, times
 //   the matrices the request keeps live at once) fit the budget; a request is always
 //   admitted when nothing else is admitted, so a request larger than the budget still
-//   runs (alone).  An admitted request's positions are copied to the device once and
-//   stay resident until it finishes.
+//   runs (alone).  A later request that fits may pass one that does not, but only
+//   kMaxBypass times: then nothing behind the passed-over request is admitted until it
+//   fits (at the latest once the device has drained), so a large request is not starved
+//   by a steady stream of small ones.  An admitted request's positions are copied to
+//   the device once and stay resident until it finishes.
 // * Continuous batching.  The batch loop repeatedly forms one step from every
 //   admitted log-det and resonance request: each contributes the matrices of its
 //   current stage (all tile cells; the FP64 re-evaluation cells of a mixed tile; the
@@ -56,6 +59,7 @@ struct Req {
     std::vector<int> reidx;
     std::unique_ptr<AberthRun> run;     // RESONANCE
     size_t footprint = 0;
+    int bypassed = 0;                   // later requests admitted while this one waited
     int status = 0, count = 0;
     std::string err;
     bool done = false;
@@ -109,13 +113,21 @@ void Server::finish_locked(Req* q) {
     cv.notify_all();
 }
 
-// Arrival order; a request that does not fit is skipped for now (later, smaller
-// requests may pass it) but is admitted first once it fits.
+// Arrival order; a request that does not fit is skipped for now and later, smaller
+// requests may pass it, at most kMaxBypass times; after that it holds the queue.
+constexpr int kMaxBypass = 4;
+
 void Server::admit_locked() {
+    std::vector<Req*> passed;           // waiting requests ahead of the current one
     for (auto it = pending.begin(); it != pending.end();) {
         Req* q = *it;
         const bool fits = in_use + q->footprint <= budget || n_admitted == 0;
-        if (!fits || n_admitted >= max_active) { ++it; continue; }
+        if (!fits || n_admitted >= max_active) {
+            if (q->bypassed >= kMaxBypass) break;      // it has waited long enough: hold the rest
+            passed.push_back(q);
+            ++it;
+            continue;
+        }
         if (upload_positions(q->mv, &q->dpos, &q->dtok) < 0) {
             q->status = -2;
             q->err = last_error();
@@ -125,6 +137,7 @@ void Server::admit_locked() {
             cv.notify_all();
             continue;
         }
+        for (Req* w : passed) w->bypassed++;
         q->mv.dpos = q->dpos;
         q->t_admit = clk::now();
         in_use += q->footprint;

@@ -734,14 +734,15 @@ typedef struct {
     int model;
     const double* p;
     orc_cplx* m;      /* n x n work matrix */
-    orc_cplx* vec;    /* the shared inverse-iteration vector (mineigval) */
+    orc_cplx* vec;    /* the inverse-iteration vector (mineigval) */
+    orc_cplx* vsave;  /* method 3: the start's vector before its first three points */
 } MullerCtx;
 
 /* f(k) = the eigenvalue reached by inverse iteration (50 steps, 1e-10) from the shared
  * vector, which keeps the last result (method 1); or 1 / tr M(k)^-1 (method 2). */
 static orc_cplx muller_f(MullerCtx* c, orc_cplx k) {
     orc_build_matrix(c->d, c->n, c->pos, c->model, c->p, k, c->m);
-    if (c->method == 1) {
+    if (c->method == 1 || c->method == 3) {
         orc_cplx mu;
         orc_inverse_iteration(c->n, c->m, c->vec, &mu, 50, 1e-10);
         return mu;
@@ -759,7 +760,19 @@ static int muller_one(MullerCtx* c, double xmin, double xmax, double ymin, doubl
     if (maxit <= 1 || toler <= 0.) return 0;
     const double h = 1e-4;                        /* relative offsets of the first two points */
     orc_cplx za = (1. - h * I) * (*z), zb = *z, zc = (1. + h * I) * (*z);
-    orc_cplx fa = muller_f(c, za), fb = muller_f(c, zb), fc = muller_f(c, zc);
+    orc_cplx fa, fb, fc;
+    if (c->method == 3) {     /* the three first points all start from the start's vector */
+        memcpy(c->vsave, c->vec, sizeof(orc_cplx) * c->n);
+        fa = muller_f(c, za);
+        memcpy(c->vec, c->vsave, sizeof(orc_cplx) * c->n);
+        fb = muller_f(c, zb);
+        memcpy(c->vec, c->vsave, sizeof(orc_cplx) * c->n);
+        fc = muller_f(c, zc);
+    } else {
+        fa = muller_f(c, za);
+        fb = muller_f(c, zb);
+        fc = muller_f(c, zc);
+    }
     double absfc = cabs(fc);
     for (int s = 0; s < maxit; s++) {
         /* divided differences; the quadratic through the three points, taken at zc */
@@ -792,14 +805,17 @@ static int muller_one(MullerCtx* c, double xmin, double xmax, double ymin, doubl
 
 int orc_roots_muller(int d, int n, const double* pos, int model, const double* p, double xmin, double xmax,
                      double ymin, double ymax, int* nz, orc_cplx* z, int maxit, double toler, int method) {
-    MullerCtx c = {d, n, method, pos, model, p, NULL, NULL};
+    MullerCtx c = {d, n, method, pos, model, p, NULL, NULL, NULL};
     c.m = (orc_cplx*)malloc(sizeof(orc_cplx) * (size_t)n * n);
     c.vec = (orc_cplx*)malloc(sizeof(orc_cplx) * n);
+    c.vsave = (orc_cplx*)malloc(sizeof(orc_cplx) * n);
     for (int i = 0; i < n; i++) c.vec[i] = 1. / sqrt((double)n);      /* constant_unit_cvector */
     int* conv = (int*)calloc(*nz, sizeof(int));
     for (int i = 0; i < *nz; i++) {
+        if (method == 3)                     /* each start from the constant unit vector */
+            for (int j = 0; j < n; j++) c.vec[j] = 1. / sqrt((double)n);
         conv[i] = muller_one(&c, xmin, xmax, ymin, ymax, &z[i], maxit, toler);
-        if (method == 1) conv[i] *= (cabs(muller_f(&c, z[i])) < sqrt(toler));
+        if (method != 2) conv[i] *= (cabs(muller_f(&c, z[i])) < sqrt(toler));
     }
     /* keep the converged roots in order, then the first of each group closer than 1e-14 */
     int size = 0;
@@ -813,6 +829,6 @@ int orc_roots_muller(int d, int n, const double* pos, int model, const double* p
         if (uniq) { orc_cplx t = z[i]; z[i] = z[u]; z[u] = t; u++; }
     }
     *nz = u;
-    free(c.m); free(c.vec); free(conv);
+    free(c.m); free(c.vec); free(c.vsave); free(conv);
     return u;
 }

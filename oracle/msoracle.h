@@ -111,7 +111,11 @@ void orc_aberth(int d, int n, const double* pos, int model, const double* p,
  * eigenvalue of M(k) by inverse iteration from one vector shared by the starts in turn
  * (method 1, find_eigenstates_serial_2, ms_core.c:335-367) or on 1/tr M(k)^-1 (method 2,
  * find_eigenstates_serial_3, ms_core.c:374-401).  The starts z[0..*nz) are run in order;
- * on exit the converged, distinct (1e-14) roots in that order.  Returns their count. */
+ * on exit the converged, distinct (1e-14) roots in that order.  Returns their count.
+ * Method 3 is method 1 with one vector per start instead of the shared one: every start
+ * begins from the constant unit vector, its first three points (za, zb, zc) all start
+ * from it, and each later evaluation from the vector of that start's previous point
+ * (the order in which the batched msgpu_roots_muller evaluates, all starts at once). */
 int orc_roots_muller(int d, int n, const double* pos, int model, const double* p, double xmin, double xmax,
                      double ymin, double ymax, int* nz, orc_cplx* z, int maxit, double toler, int method);
 

@@ -289,7 +289,7 @@ def inverse_iteration(m, v0, maxit, toler):
     return conv, v, complex(lam[0])
 
 
-MINEIGVAL, INVTRACEINV = 1, 2
+MINEIGVAL, INVTRACEINV, MINEIGVAL_PER_START = 1, 2, 3
 
 
 def roots_muller(pos, starts, dom, method, model=HARDSPHERE, p=(0.1,), maxit=30, toler=1e-10):

@@ -36,15 +36,32 @@ def medium(G, pos, model=HS, p=(0.1,)):
     return m
 
 
-def tl_close(got, want, n, tol=1e-10, argtol=1e-10):
+def tl_close(got, want, n, tol=1e-10, argtol=1e-10, floor=None):
     """tracelog = log det / N: Re to tol * max(1, |Re|); arg det = N Im modulo 2 pi to argtol
-    absolute (radians).  The achieved errors are printed."""
+    absolute (radians).  floor (optional, per cell): (Re, arg) disagreement of two
+    independent FP64 CPU factorizations of the same matrix; a cell may then differ by up
+    to 10 times that instead (matrices with condition numbers near 1e21-1e31, where FP64
+    LU itself cannot fix arg det to 1e-10).  The achieved errors are printed."""
     got, want = np.asarray(got), np.asarray(want)
-    ere = np.max(np.abs(got.real - want.real) / np.maximum(1, np.abs(want.real)))
-    earg = np.max(np.abs(wrapped(n * (got.imag - want.imag))))
-    print("N=%d, %d cells: max rel |d Re tracelog| %.2e, max |d arg det| %.2e rad" % (n, got.size, ere, earg))
-    assert ere <= tol, ere
-    assert earg <= argtol, earg
+    dre = np.abs(got.real - want.real) / np.maximum(1, np.abs(want.real))
+    darg = np.abs(wrapped(n * (got.imag - want.imag)))
+    tre, targ = np.full(got.shape, tol), np.full(got.shape, argtol)
+    if floor is not None:
+        fl = np.asarray(floor)
+        tre, targ = np.maximum(tre, 10 * fl[:, 0]), np.maximum(targ, 10 * fl[:, 1])
+    print("N=%d, %d cells: max rel |d Re tracelog| %.2e, max |d arg det| %.2e rad (tolerance %.1e)"
+          % (n, got.size, dre.max(), darg.max(), targ.max()))
+    assert (dre <= tre).all(), (dre, tre)
+    assert (darg <= targ).all(), (darg, targ)
+
+
+def lapack_floor(m, want, n):
+    """(Re, arg det) distance between the reference value and numpy's LAPACK slogdet of the
+    same matrix, and between the reference value and the oracle's LU."""
+    sgn, ld = np.linalg.slogdet(m)
+    o = O.tracelog(m)
+    return (max(abs(ld / n - want.real) / max(1, abs(want.real)), abs(o.real - want.real) / max(1, abs(want.real))),
+            max(abs(wrapped(np.angle(sgn) - n * want.imag)), abs(wrapped(n * (o.imag - want.imag)))))
 
 
 # --------------------------------------------------------------------- media
@@ -121,7 +138,8 @@ def test_tracelog_lu_dropin(G):
     for k in (1 - 1j, 3 - 2j, 10 - 0.5j, 10 - 10j):
         mm = O.build_matrix(pos, k, HS, (0.4,))
         t, lu = G.tracelog_lu(mm)
-        tl_close([t], [O.tracelog(mm)], 50)
+        want = O.tracelog(mm)
+        tl_close([t], [want], 50, floor=[lapack_floor(mm, want, 50)])
 
 
 def test_tracelog_lu_concurrent_calls(G):
@@ -151,7 +169,11 @@ def test_tracelog_batch_fp64_vs_reference(G, ref, tag, p, n):
     m = medium(G, pos, HS, (p,))
     ks = ref["tl_%s_k" % tag]
     out, flags = G.tracelog_batch(m, ks, G.FP64)
-    tl_close(out, ref["tl_%s_val" % tag], n)
+    want = ref["tl_%s_val" % tag]
+    # the reference test's k = 6-6i and 10-10i (d=2 N=50) have condition numbers near 1e21
+    # and 1e31: there the tolerance follows the CPU factorizations' own disagreement
+    floor = [lapack_floor(O.build_matrix(pos, k, HS, (p,)), w, n) for k, w in zip(ks, want)] if tag == "test" else None
+    tl_close(out, want, n, floor=floor)
 
 
 def test_tracelog_batch_many_k_vs_oracle(G, ref):
@@ -218,6 +240,26 @@ def test_mixed_solve_refines_to_fp64(G, ref):
     a32, steps = G.solve(m, 10.0, G.PLANE, G.MIXED, 1e-12)
     assert steps >= 1
     assert np.abs(a32 - a64).max() < 1e-10 * np.abs(a64).max()
+
+
+def test_mixed_solve_at_a_resonance_falls_back_to_fp64(G, ref):
+    """At a root of det M (cond M ~ 1e12, far beyond 1/eps32) complex64 refinement stalls:
+    the solve must not return its unconverged iterate but the complex128 solution, whose
+    residual is at the level of the complex128 backward error."""
+    pos = O.fill_medium(2, 20, 1)
+    m = medium(G, pos, HS, (0.4,))
+    k = ref["ab_d2_roots"][0]
+    a64, _ = G.solve(m, k, G.PLANE, G.FP64)
+    a32, steps = G.solve(m, k, G.PLANE, G.MIXED, 1e-12)
+    assert steps >= 1
+    assert np.array_equal(a32, a64)
+    M = O.build_matrix(pos, k, HS, (0.4,))
+    phi = np.exp(1j * k * pos[:, 0])
+    res = np.linalg.norm(M @ a32 - phi)
+    print("|a| %.2e, residual %.2e, backward error %.2e" % (np.linalg.norm(a32), res,
+          res / (np.linalg.norm(M, 2) * np.linalg.norm(a32))))
+    assert np.linalg.norm(a32) > 1e6 * np.linalg.norm(phi) / np.linalg.norm(M, 2)      # truly near-singular
+    assert res <= 1e-13 * 20 * np.linalg.norm(M, 2) * np.linalg.norm(a32)
 
 
 def test_scattering_amplitudes_mathematica(G):

@@ -162,3 +162,64 @@ def test_malformed_request_rejected(G):
     with pytest.raises(G.MsgpuError):
         srv.submit_logdet(med, np.zeros(0, np.complex128))
     srv.close()
+
+
+def test_large_requests_mix_equals_alone(G):
+    """Requests of the serving config's upper N range: a d=3 N=4096 log-det tile and a d=2
+    N=4500 mixed-precision solve with a psi grid, each above the admission budget (they run
+    alone), among small requests; every result is the same alone and in the mix."""
+    rng = np.random.default_rng(40)
+    big_tile = G.Medium(3, 4096, G.HARDSPHERE, (0.1,)).fill(41)
+    big_solve = G.Medium(2, 4500, G.HARDSPHERE, (0.1,)).fill(42)
+    small = G.Medium(3, 300, G.HARDSPHERE, (0.1,)).fill(43)
+    reqs = [("logdet", big_tile, dict(ks=rng.uniform(5, 30, 8) - 1j * rng.uniform(0, 2, 8), mode=G.FP64,
+                                      reeval_rel=1e-6)),
+            ("logdet", small, dict(ks=rng.uniform(2, 10, 16) - 1j * rng.uniform(0, 1, 16), mode=G.MIXED,
+                                   reeval_rel=1e-2)),
+            ("solve", big_solve, dict(k=6.0, mode=G.MIXED, tol=1e-12, grid=(32, 24, -30.0, 30.0, -25.0, 25.0))),
+            ("solve", small, dict(k=5.0, mode=G.FP64, tol=0.0, grid=(16, 12, -5.0, 5.0, -4.0, 4.0)))]
+    budget_gb = 0.25
+    for kind, med, a in reqs[::2]:
+        assert footprint_bound(kind, med, a) > budget_gb * 2 ** 30
+    outs = []
+    for kind, med, a in reqs:
+        srv = G.Server(hbm_gb=64.0)
+        print("alone", kind, med.n, flush=True)
+        outs.append(srv.wait(submit(srv, kind, med, a))[0])
+        srv.close()
+    srv = G.Server(hbm_gb=budget_gb, max_active=8)
+    ids = [submit(srv, kind, med, a) for kind, med, a in reqs]
+    for i, want in zip(ids, outs):
+        same(srv.wait(i)[0], want)
+    srv.close()
+    # the big solve reached the refinement tolerance (or its complex128 fallback)
+    aw, steps = G.solve(big_solve, 6.0, G.PLANE, G.MIXED, 1e-12)
+    assert np.array_equal(outs[2]["a"], aw) and steps >= 1
+    assert np.isfinite(outs[0]["tracelog"]).all()
+
+
+def test_large_request_not_starved(G):
+    """A request larger than the budget behind a steady stream of small ones: it is passed
+    over a bounded number of times, then holds the queue until the device has drained and
+    runs alone."""
+    small = G.Medium(3, 200, G.HARDSPHERE, (0.1,)).fill(51)
+    big = G.Medium(3, 1000, G.HARDSPHERE, (0.1,)).fill(52)
+    ks_small = np.linspace(3.0, 9.0, 24) - 0.3j          # 24 x 16 x 200^2 B = 15 MB each
+    ks_big = np.linspace(5.0, 25.0, 8) - 0.5j             # 128 MB, above the 50 MB budget
+    srv = G.Server(hbm_gb=50e6 / 2 ** 30, max_active=8)
+    t_sub, ids = [], []
+    for j in range(43):
+        if j == 3:
+            t_big = time.perf_counter()
+            big_id = srv.submit_logdet(big, ks_big, G.FP64)
+        t_sub.append(time.perf_counter())
+        ids.append(srv.submit_logdet(small, ks_small, G.FP64))
+        time.sleep(0.001)
+    admit = [t + srv.wait(i)[1]["queue_ms"] * 1e-3 for t, i in zip(t_sub, ids)]
+    out_big, info_big = srv.wait(big_id)
+    srv.close()
+    admit_big = t_big + info_big["queue_ms"] * 1e-3
+    passed = sum(1 for j in range(3, 43) if admit[j] < admit_big)
+    print("small requests submitted after the large one and admitted before it:", passed)
+    assert passed <= 6, passed                           # at most 4 bypasses (+ clock slack)
+    assert np.isfinite(out_big["tracelog"]).all()
