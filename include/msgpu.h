@@ -149,6 +149,16 @@ int msgpu_tune_load(const char* path);
  * error (message in msgpu_last_error). */
 int msgpu_tracelog_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode,
                          double reeval_rel, int* flags);
+/* Allocation-free variant of msgpu_tracelog_batch, mode MSGPU_FP64 or MSGPU_FP32.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP-graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc/hipFree or allocating library routine: every device buffer lives
+ * in one static 256 MiB device array per GPU, held until the process ends.  Limit: one
+ * matrix and its workspace must fit in it, N <= 4032 (complex128) or N <= 5760
+ * (complex64); larger N returns -5 with a message.  Calls serialise on one lock per
+ * process (they share each GPU's array).  Results equal msgpu_tracelog_batch's bit for
+ * bit.  Returns 0, or < 0 on error (message in msgpu_last_error). */
+int msgpu_tracelog_batch_noalloc(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode);
 /* As msgpu_tracelog_batch in FP64 or FP32 (no re-evaluation), with the pivot statistics
  * stats[2l] = max_i log|U_ii|, stats[2l+1] = min_i log|U_ii| of cell l.  No counterpart in
  * the reference; a conditioning indicator for the caller. */

@@ -84,6 +84,7 @@ _SIG = {
     "msgpu_profile_end": (ctypes.c_int, [_dp, _dp, _dp, _ip, ctypes.c_int]),
     "msgpu_tracelog_batch": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, _ip]),
     "msgpu_dk_log_det_batch": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
+    "msgpu_tracelog_batch_noalloc": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, ctypes.c_int]),
     "msgpu_tracelog_stats": (ctypes.c_int, [_mp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     "msgpu_solve": (ctypes.c_int, [_mp, ctypes.POINTER(IWave), _dp, ctypes.c_int, ctypes.c_double]),
     "msgpu_eigenstates": (ctypes.c_int, [_mp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp, _ip,
@@ -232,6 +233,14 @@ def tracelog_batch(med, ks, mode=FP64, reeval_rel=1e-6):
     _check(lib.msgpu_tracelog_batch(med.ref, ks.size, _cp(ks), _cp(out), mode, reeval_rel,
                                     flags.ctypes.data_as(_ip)), "msgpu_tracelog_batch")
     return out, flags
+
+
+def tracelog_batch_noalloc(med, ks, mode=FP64):
+    """msgpu_tracelog_batch_noalloc: the trace-log map without device allocation (FP64/FP32)."""
+    ks = np.ascontiguousarray(ks, dtype=np.complex128)
+    out = np.zeros_like(ks)
+    _check(lib.msgpu_tracelog_batch_noalloc(med.ref, ks.size, _cp(ks), _cp(out), mode), "msgpu_tracelog_batch_noalloc")
+    return out
 
 
 def tracelog_stats(med, ks, mode=FP64):

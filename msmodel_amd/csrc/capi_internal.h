@@ -49,6 +49,8 @@ inline int padded(int n) { return (n + 63) / 64 * 64; }
 void set_error(const std::string& msg);
 const char* last_error();
 int engine_init();
+// noalloc.hip: the device address of the allocation-free scratch on the current device
+int noalloc_setup();
 int profile_begin();
 int profile_end(double* ms, double* flops, double* bytes, int* count, int ncat);
 

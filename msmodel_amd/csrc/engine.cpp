@@ -197,6 +197,7 @@ int engine_init() {
         }
         g_init_rc = 0;
         load_initial_tuning();
+        (void)noalloc_setup();      // a failure there is reported by the allocation-free entry point
     });
     if (g_init_rc != 0) set_error("msgpu: no usable HIP device (hipGetDeviceCount/hipSetDevice failed)");
     return g_init_rc;
@@ -226,9 +227,12 @@ static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
 
+// Launches of the allocation-free entry point are never timed (no events are created).
+static thread_local bool t_untimed = false;
+
 template <class F>
 static void timed(int cat, double flops, double bytes, hipStream_t st, F&& launch) {
-    if (!g_prof_on) { launch(); return; }
+    if (!g_prof_on || t_untimed) { launch(); return; }
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
@@ -373,6 +377,25 @@ static void factor(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* p
     if (tune_for(np, prec_of<T>()).blk2) factor2<T>(mats, np, nbatch, ipiv, info, pmap, left, rm, st, after_first_panel);
     else factor1<T>(mats, np, nbatch, ipiv, info, pmap, left, rm, st, after_first_panel);
 }
+
+// The blocked LU of a batch for the allocation-free entry point (noalloc.hip): the
+// tuning table's blocking, storage order and launch shapes for np, no timing events.
+// Returns the storage order used (1 row-major).
+template <class T>
+int factor_untimed(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* ws, hipStream_t st) {
+    const bool rm = tune_for(np, prec_of<T>()).rm == 1;
+    t_untimed = true;
+    try {
+        factor<T>(mats, np, nbatch, ipiv, info, ws, false, rm, st);
+    } catch (...) {
+        t_untimed = false;
+        throw;
+    }
+    t_untimed = false;
+    return rm ? 1 : 0;
+}
+template int factor_untimed<double>(cx<double>*, int, int, int*, int*, int*, hipStream_t);
+template int factor_untimed<float>(cx<float>*, int, int, int*, int*, int*, hipStream_t);
 
 template <class T>
 static void factor1(cx<T>* mats, int np, int nbatch, int* ipiv, int* info, int* pmap, bool left, bool rm,
