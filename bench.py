@@ -88,11 +88,30 @@ def run_msmain(exe, n, nx, ny, threads):
 
 
 def host_cores():
+    """Threads for the host legs: the CPU share the process may use (OMP_NUM_THREADS when
+    set, as on a GPU box whose share of the host is fixed per GPU), else every core in
+    the affinity mask."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    return max(1, min(cores, 16))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        cores = min(cores, int(share))
+    return max(1, cores)
+
+
+def host_description():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "cpus_online": os.cpu_count(),
+            "lapack": "Intel MKL via /opt/conda/lib/libmkl_rt.so.1 (sequential per thread)"}
 
 
 def cpu_baseline(n, nx, ny):
@@ -104,10 +123,12 @@ def cpu_baseline(n, nx, ny):
     try:
         rt = run_msmain(exe, n, nx, ny, cores)
         evals = nx * ny
-        return {"value": evals / rt, "unit": "log-det evals/s", "cores": cores, "kind": "reference",
+        line = {"value": evals / rt, "unit": "log-det evals/s", "cores": cores, "kind": "reference",
                 "sample": "msmain -p %d (gcc -O2, MKL sequential per thread): d=3 N=%d unit-density ball seed 1, "
                           "hardsphere 0.05, %dx%d k-map over Re[5,30] x Im[-2,0] = %d log-dets in %.2f s"
                           % (cores, n, nx, ny, evals, rt)}
+        line.update(host_description())
+        return line
     except Exception as ex:  # the baseline is reported, never fatal to the GPU measurement
         return {"value": None, "unit": "log-det evals/s", "cores": cores, "kind": "reference",
                 "sample": "failed: %s" % ex}
@@ -189,24 +210,33 @@ def serving_stream(G, nreq, rate, nmin, nmax, seed=5):
             "max_requests_per_step": st["max_requests_per_step"]}
 
 
-def uneven_search(G, nmed, per, nmin, nmax):
-    """Resonance search over media of uneven size in one call (msgpu_resonances_multi):
-    d=2, nmed media (seeds 100..) with N log-uniform in [nmin, nmax] in unit disks,
-    alpha=0.1, per starts each uniform in Re k in [1,40] x Im k in [-1,0], complex64
-    sweeps then complex128 polish to 1e-12."""
+def uneven_media(G, nmed, total, nmin, nmax):
+    """The uneven-batch resonance search's media and starts: d=2, nmed media (seeds 100..)
+    with N log-uniform in [nmin, nmax] in unit disks, alpha=0.1, total starts uniform in
+    Re k in [1,40] x Im k in [-1,0], as evenly spread over the media as they divide."""
     rng = np.random.default_rng(100)
     meds, starts, doms = [], [], []
-    for s in range(100, 100 + nmed):
+    for i, s in enumerate(range(100, 100 + nmed)):
         n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        per = total // nmed + (1 if i < total % nmed else 0)
         meds.append(G.Medium(2, n, G.HARDSPHERE, (0.1,)).fill(s, radius=1.0))
         starts.append(rng.uniform(1.0, 40.0, per) + 1j * rng.uniform(-1.0, 0.0, per))
         doms.append((1.0, 40.0, -1.0, 0.0))
+    return meds, starts, doms
+
+
+def uneven_search(G, nmed, total, nmin, nmax):
+    """Resonance search over media of uneven size in one call (msgpu_resonances_multi):
+    complex64 sweeps then complex128 polish to |dk| < 1e-12 |k|."""
+    meds, starts, doms = uneven_media(G, nmed, total, nmin, nmax)
     t0 = time.perf_counter()
     roots, sweeps = G.resonances_multi(meds, starts, doms, 4, 40, 1e-12, G.MIXED)
     dt = time.perf_counter() - t0
-    return {"media": nmed, "starts": nmed * per, "N_range": [nmin, nmax], "sweeps": sweeps, "wall_s": dt,
-            "roots_found": int(sum(len(r) for r in roots)),
-            "sample": "%d media x %d starts (the full search runs 10000 starts)" % (nmed, per)}
+    ns = np.array([m.n for m in meds])
+    return {"media": nmed, "starts": int(sum(len(z) for z in starts)), "N_range": [nmin, nmax],
+            "N_of_media_min_median_max": [int(ns.min()), int(np.median(ns)), int(ns.max())],
+            "sweeps": sweeps, "wall_s": dt, "roots_found": int(sum(len(r) for r in roots)),
+            "starts_per_s": sum(len(z) for z in starts) / dt}
 
 
 def tune_entry(G, np_, prec):
@@ -262,6 +292,26 @@ def small_map(G, steps, tune):
             "launch_shapes": tuned}
 
 
+def serialized_pass(G, med, ks, np_):
+    """One pass of the map on a single stream (the tuning entry of np_ with streams=1),
+    every launch bracketed by HIP events on that stream: the kernels run one after the
+    other, so each launch's event interval is its duration alone on the GPU."""
+    tuned = tune_entry(G, np_, 0)
+    if tuned is None:
+        return None
+    set_entry(G, np_, 0, dict(tuned, streams=1))
+    try:
+        G.tracelog_batch(med, ks, G.FP64)
+        G.profile_begin()
+        t0 = time.perf_counter()
+        G.tracelog_batch(med, ks, G.FP64)
+        wall = time.perf_counter() - t0
+        prof = G.profile_end()
+    finally:
+        set_entry(G, np_, 0, tuned)
+    return prof, wall
+
+
 def progress(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -299,7 +349,7 @@ def main():
     ap.add_argument("--rate", type=float, default=8.0, help="Poisson arrival rate of the serving stream (1/s)")
     ap.add_argument("--serve-nmax", type=int, default=2000, help="largest N of the serving stream")
     ap.add_argument("--uneven", type=int, default=64, help="media of the uneven resonance search (0: skip)")
-    ap.add_argument("--uneven-starts", type=int, default=4, help="starts per medium of the uneven search")
+    ap.add_argument("--uneven-starts", type=int, default=10000, help="starts of the uneven search (all media)")
     ap.add_argument("--no-wave", action="store_true", help="skip the solve + 1024^2 psi case")
     ap.add_argument("--tune", action="store_true", help="run the autotuner for N before timing")
     ap.add_argument("--tune-file", default="", help="with --tune: where to save the tuning table")
@@ -307,6 +357,8 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the N=200 map (eager vs HIP graph)")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the serialized pass of the map (one stream), for the rocprof kernel-trace run")
     ap.add_argument("--no-dropin", action="store_true", help="skip timing msmain relinked against the library")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run this many device copies of 1 GiB (calibrates the HBM byte counters)")
@@ -350,6 +402,17 @@ def main():
         tuned = {"table": G.tuning_table_path()}
     progress("launch shapes", tuned)
 
+    if args.roofline_only:
+        np_ = (n + 63) // 64 * 64
+        prof, wall = serialized_pass(G, med, ks, np_)     # its warm-up pass is serialized too
+        g = prof["gemm"]
+        print(json.dumps({"serialized_pass_ms": 1e3 * wall, "gemm_launches": g["launches"],
+                          "gemm_avg_ms": g["ms"] / max(1, g["launches"]),
+                          "gemm_tflops": g["flops"] / max(1e-12, g["ms"] * 1e-3) / 1e12,
+                          "kernel_ms": {c: v["ms"] for c, v in prof.items()},
+                          "launches": {c: v["launches"] for c, v in prof.items()}}))
+        return
+
     def timed_path(mode):
         for _ in range(args.warmup):
             G.tracelog_batch(med, ks, mode, 1e-6)
@@ -391,6 +454,7 @@ def main():
         progress("serving stream done")
     if args.uneven > 0:
         uneven = uneven_search(G, args.uneven, args.uneven_starts, 200, 2000)
+        progress("uneven search: %d starts in %.1f s" % (uneven["starts"], uneven["wall_s"]))
         progress("uneven resonance search done")
     if not args.no_wave:
         wave = wavefunction_case(G)
@@ -407,22 +471,32 @@ def main():
         value, value_path, prof, peak, kname = fp64_batched, "fp64_batched", prof64, PEAK_FP64_MFMA, \
             "gemm_sub_kernel<double>"
         ms_step = 1e3 * t64 / args.steps
+    # The dominant kernel's roofline from a serialized pass (one stream: each launch alone on
+    # the GPU, durations from HIP events on its stream), measured here; the same launches in
+    # the timed region (four streams sharing the GPU, which stretches each launch's event
+    # interval) are reported beside it as *_in_mix.
     g = prof["gemm"]
-    achieved = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
+    ser = serialized_pass(G, med, ks, (n + 63) // 64 * 64) if value_path == "fp64_batched" else None
+    gs = ser[0]["gemm"] if ser else g
+    achieved = gs["flops"] / (gs["ms"] * 1e-3) if gs["ms"] > 0 else 0.0
+    achieved_mix = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": measured_traffic(kname),
+                "traffic_source": "profiles/traffic.json, from the counter runs of profiles/pmc_lu_map_n1000.md "
+                                  "(rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE per launch, scaled by a 1 GiB copy)",
                 "kernel": kname + " (LU trailing update, MFMA 16x16x4)",
-                "launches": g["launches"], "avg_ms": g["ms"] / max(1, g["launches"]),
-                "algorithmic_bytes": g["bytes"] / max(1, g["launches"])}
-    # the same launches alone on the GPU (counter run): the streams' kernels share the GPU
-    # in the timed region, which stretches every launch's duration; reported when this
-    # run's launches are the recorded ones (same flops per launch)
-    rec = counter_record(kname)
-    if rec and g["launches"] and "gflop_per_launch" in rec:
-        fl = g["flops"] / g["launches"]
-        if abs(fl / (rec["gflop_per_launch"] * 1e9) - 1) < 0.01:
-            roofline["avg_ms_alone"] = rec["avg_us_alone"] * 1e-3
-            roofline["frac_alone"] = fl / (rec["avg_us_alone"] * 1e-6) / peak
+                "timing": "serialized pass: one stream, HIP events around every launch" if ser else
+                          "timed region (four streams)",
+                "launches": gs["launches"], "avg_ms": gs["ms"] / max(1, gs["launches"]),
+                "flops_per_launch": gs["flops"] / max(1, gs["launches"]),
+                "algorithmic_bytes": gs["bytes"] / max(1, gs["launches"]),
+                "serialized_pass_evals_per_s": nk / ser[1] if ser else None,
+                "achieved_in_mix": achieved_mix / 1e12, "frac_in_mix": achieved_mix / peak,
+                "avg_ms_in_mix": g["ms"] / max(1, g["launches"]), "launches_in_mix": g["launches"]}
+    if ser:
+        tot = sum(v["ms"] for v in ser[0].values())
+        roofline["serialized_kernel_ms"] = {c: round(v["ms"], 3) for c, v in ser[0].items()}
+        roofline["serialized_kernel_fraction"] = {c: round(v["ms"] / max(1e-9, tot), 4) for c, v in ser[0].items()}
     roofline_c64 = None
     if prof32 is not None:
         g32 = prof32["gemm"]
@@ -449,6 +523,8 @@ def main():
         "fp64_plain_evals_per_s": fp64_plain, "value_over_fp64_plain": value / fp64_plain if fp64_plain else None,
         "mixed_reevaluated_cells": int(flags.sum()), "mixed_reeval_rel": 1e-6,
         "mixed_max_abs_dtracelog": float(dtl.max()), "mixed_median_abs_dtracelog": float(np.median(dtl)),
+        "mixed_p90_p99_abs_dtracelog": [float(np.percentile(dtl, 90)), float(np.percentile(dtl, 99))],
+        "mixed_cells_within_1e-8": int((dtl <= 1e-8).sum()),
         "mixed_max_abs_dtracelog_re": dtl_re, "roofline_c64_gemm": roofline_c64,
         "resonance_latency_ms": serving["by_kind_ms"]["resonance32"] if serving else None,
         "serving": serving,

@@ -243,6 +243,15 @@ int msgpu_resonances_multi(int nmed, const msgpu_medium* const* meds, const doub
 int msgpu_resonances(const msgpu_medium* med, double xmin, double xmax, double ymin, double ymax, int* nz,
                      msgpu_cplx* z, int nexpsup, int maxit, double toler, int mode);
 
+/* k-point generators of the reference's k-plane tasks, bit for bit: the nx*ny points of a
+ * complex map (get_arg_cmap, complex_map_util.c:27-31), the 3(ns+2) points of a complex
+ * cut (get_arg_ccut, complex_cut_util.c:32-36) and the ns points of a log cut
+ * (init_logcut, log_cut_util.c:72-101), in their storage order.  Each returns the number
+ * of points written, or < 0 on bad arguments. */
+int msgpu_kgrid_map(int nx, int ny, double xmin, double xmax, double ymin, double ymax, msgpu_cplx* k);
+int msgpu_kgrid_cut(int ns, msgpu_cplx zmin, msgpu_cplx zmax, msgpu_cplx* k);
+int msgpu_kgrid_logcut(int ns, double rek, double imkmin, double imkmax, msgpu_cplx* k);
+
 /* ------------------------------------------------------- 3. serving loop */
 /* No counterpart in the reference (msmain runs one task at a time).  A server admits
  * log-det tiles, Foldy-Lax solves (+ psi grid) and resonance searches of mixed N and d

@@ -105,6 +105,9 @@ _SIG = {
     "msgpu_resonances_multi": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_mp), _dp, _ip, _dp, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "msgpu_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    "msgpu_kgrid_map": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [_dp]),
+    "msgpu_kgrid_cut": (ctypes.c_int, [ctypes.c_int, Cplx, Cplx, _dp]),
+    "msgpu_kgrid_logcut": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
     "msgpu_tune_save": (ctypes.c_int, [ctypes.c_char_p]),
     "msgpu_tune_load": (ctypes.c_int, [ctypes.c_char_p]),
     "msgpu_server_create": (ctypes.c_void_p, [ctypes.c_double, ctypes.c_int]),
@@ -410,6 +413,24 @@ def zgetrs(lu, ipiv, b):
 
 
 # ------------------------------------------------------------------ autotuner
+def kgrid_map(nx, ny, xmin, xmax, ymin, ymax):
+    k = np.zeros(nx * ny, dtype=np.complex128)
+    _check(lib.msgpu_kgrid_map(nx, ny, xmin, xmax, ymin, ymax, _cp(k)), "msgpu_kgrid_map")
+    return k
+
+
+def kgrid_cut(ns, zmin, zmax):
+    k = np.zeros(3 * (ns + 2), dtype=np.complex128)
+    _check(lib.msgpu_kgrid_cut(ns, _c(zmin), _c(zmax), _cp(k)), "msgpu_kgrid_cut")
+    return k
+
+
+def kgrid_logcut(ns, rek, imkmin, imkmax):
+    k = np.zeros(ns, dtype=np.complex128)
+    _check(lib.msgpu_kgrid_logcut(ns, rek, imkmin, imkmax, _cp(k)), "msgpu_kgrid_logcut")
+    return k
+
+
 def autotune(n, mode=FP64, nitems=512):
     """Pick the fastest swap + TRSM shape, chunk size and stream count for size n."""
     rep = np.zeros(8)

@@ -397,6 +397,48 @@ MSGPU_API int msgpu_resonances(const msgpu_medium* med, double xmin, double xmax
     return resonances_any(views_of(1, &med), dom, nz, z, nexpsup, maxit, toler, mode);
 }
 
+// ------------------------------------------------------------------ k-point generators
+// The sample points of the reference's k-plane tasks, in their storage order, so that a
+// caller can hand a whole map, cut or log cut to the batched entry points.  The
+// arithmetic is that of the reference's generators, operation for operation (the
+// results are equal bit for bit).
+// get_arg_cmap (complex_map_util.c:27-31, 90-91): row-major from the top-left corner.
+MSGPU_API int msgpu_kgrid_map(int nx, int ny, double xmin, double xmax, double ymin, double ymax, msgpu_cplx* k) {
+    if (nx < 2 || ny < 2 || !k) { set_error("msgpu_kgrid_map: nx, ny >= 2 and an output array needed"); return -2; }
+    const double dx = (xmax - xmin) / (nx - 1), dy = (ymax - ymin) / (ny - 1);
+    for (int l = 0; l < nx * ny; l++) {
+        const int ix = l % nx, iy = l / nx;
+        k[l] = msgpu_cplx(xmin + ix * dx, ymax - iy * dy);
+    }
+    return nx * ny;
+}
+
+// get_arg_ccut (complex_cut_util.c:32-36, 68): 3 (ns + 2) points, triplet-major, each
+// triplet across the cut (offsets -i dz, 0, +i dz with the complex step dz).
+MSGPU_API int msgpu_kgrid_cut(int ns, msgpu_cplx zmin, msgpu_cplx zmax, msgpu_cplx* k) {
+    if (ns < 2 || !k) { set_error("msgpu_kgrid_cut: ns >= 2 and an output array needed"); return -2; }
+    const double dzr = (zmax.real() - zmin.real()) / (ns - 1), dzi = (zmax.imag() - zmin.imag()) / (ns - 1);
+    for (int l = 0; l < 3 * (ns + 2); l++) {
+        const int il = l / 3, it = l % 3;
+        const double re = ((ns - il) * zmin.real() + (il - 1) * zmax.real()) / (ns - 1);
+        const double im = ((ns - il) * zmin.imag() + (il - 1) * zmax.imag()) / (ns - 1);
+        k[l] = msgpu_cplx(re + (-dzi) * (it - 1), im + dzr * (it - 1));      // + i dz (it - 1)
+    }
+    return 3 * (ns + 2);
+}
+
+// init_logcut (log_cut_util.c:72-101): rek + i imkmin (imkmax/imkmin)^(i/(ns-1)), computed as
+// imkmin exp(i log(imkmax/imkmin) / (ns - 1)).
+MSGPU_API int msgpu_kgrid_logcut(int ns, double rek, double imkmin, double imkmax, msgpu_cplx* k) {
+    if (ns < 2 || !k || imkmin >= 0 || imkmax >= 0) {
+        set_error("msgpu_kgrid_logcut: ns >= 2, negative end points and an output array needed");
+        return -2;
+    }
+    const double lfac = std::log(imkmax / imkmin);
+    for (int i = 0; i < ns; i++) k[i] = msgpu_cplx(rek, imkmin * std::exp((lfac * i) / (ns - 1)));
+    return ns;
+}
+
 // ------------------------------------------------------------------ drop-in: ms_matrix_util.h
 MSGPU_API void build_ms_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* matrix) {
     die_on(build_matrix_host(view_of(med), to_cx(k), 0, as_cx(matrix)), "build_ms_matrix");

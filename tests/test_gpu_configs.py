@@ -94,21 +94,47 @@ def test_map_n500_mixed(G):
         assert abs(f64[l].real - want.real) <= 1e-10 * max(1.0, abs(want.real)), l
 
 
-def test_uneven_resonance_search(G):
+def uneven_media(G, nmed=64, per=4):
+    """The uneven-batch resonance search: d=2, nmed media (seeds 100..) with N log-uniform in
+    [200, 2000] in unit disks, alpha=0.1, per starts each uniform in Re k in [1,40] x Im k in
+    [-1,0] (the configuration's 10000 starts are the bench's; the test takes 4 per medium)."""
     rng = np.random.default_rng(100)
     meds, poss, starts, doms = [], [], [], []
-    for s in range(100, 104):
-        n = int(round(np.exp(rng.uniform(np.log(200), np.log(400)))))
+    for s in range(100, 100 + nmed):
+        n = int(round(np.exp(rng.uniform(np.log(200), np.log(2000)))))
         meds.append(G.Medium(2, n, G.HARDSPHERE, (0.1,)).fill(s, radius=1.0))
-        poss.append(scaled(2, n, s, 1.0))
-        starts.append(rng.uniform(1.0, 40.0, 4) + 1j * rng.uniform(-1.0, 0.0, 4))
+        poss.append(meds[-1].pos)
+        starts.append(rng.uniform(1.0, 40.0, per) + 1j * rng.uniform(-1.0, 0.0, per))
         doms.append((1.0, 40.0, -1.0, 0.0))
-    roots, sweeps = G.resonances_multi(meds, starts, doms, 4, 40, 1e-12, G.MIXED)
-    assert sweeps > 0
-    found = 0
-    for pos, r in zip(poss, roots):
-        for z in r:
-            step = 1.0 / O.dk_log_det(pos, z, HS, (0.1,))
-            assert abs(step) <= 1e-10 * abs(z), (z, step)
-            found += 1
-    assert found >= 1
+    return meds, poss, starts, doms
+
+
+def test_uneven_resonance_search(G):
+    """64 media of N log-uniform in [200, 2000], 4 starts each, in one call: the mixed search
+    (complex64 sweeps, complex128 polish) finds the same root set as the complex128 search
+    in every medium (1e-10 |k|), and every root is a zero of det M: a complex128 Newton step
+    on log det stays below 1e-10 |k| (the oracle's step for the media up to N = 260)."""
+    meds, poss, starts, doms = uneven_media(G)
+    print("media N:", sorted(m.n for m in meds), flush=True)
+    r64, sw64 = G.resonances_multi(meds, starts, doms, 4, 40, 1e-12, G.FP64)
+    r32, sw32 = G.resonances_multi(meds, starts, doms, 4, 40, 1e-12, G.MIXED)
+    assert sw64 > 0 and sw32 > 0
+    nroots, worst, nsmall = 0, 0.0, 0
+    for m, pos, a, b in zip(meds, poss, r64, r32):
+        assert len(a) == len(b), (m.n, np.sort_complex(a), np.sort_complex(b))
+        for z in a:
+            assert np.min(np.abs(b - z)) <= 1e-10 * abs(z), (m.n, z)
+        for z in b:
+            assert np.min(np.abs(a - z)) <= 1e-10 * abs(z), (m.n, z)
+        if len(a):
+            steps = 1.0 / G.dk_log_det_batch(m, a, G.FP64)
+            worst = max(worst, float(np.max(np.abs(steps) / np.abs(a))))
+            assert (np.abs(steps) <= 1e-10 * np.abs(a)).all(), m.n
+            if m.n <= 260:
+                for z in a:
+                    assert abs(1.0 / O.dk_log_det(pos, z, HS, (0.1,))) <= 1e-10 * abs(z), (m.n, z)
+                    nsmall += 1
+        nroots += len(a)
+    print("%d roots in 64 media (%d checked by the oracle), sweeps FP64 %d / mixed %d, worst Newton step %.1e |k|"
+          % (nroots, nsmall, sw64, sw32, worst))
+    assert nroots >= 10            # 19 with these starts (4 per medium over Re k in [1, 40])

@@ -114,3 +114,17 @@ def test_eigenstates_vs_reference(G, ref, tag):
     for i, k in enumerate(c["kroot"]):
         dup = any(abs(k - c["kroot"][j]) < 1e-6 * abs(k) for j in range(i))
         assert bool(suspect[i]) == (dup or abs(c["mu"][i]) > 1e-6), i
+
+
+def test_kgrids_bit_exact(G, ref):
+    """msgpu_kgrid_map/_cut/_logcut against the reference's get_arg_cmap, get_arg_ccut and
+    init_logcut + get_arg_lcut (complex_map_util.c:27-31, complex_cut_util.c:32-36,
+    log_cut_util.c:72-101), bit for bit."""
+    for i, g in enumerate(ref["grid_map_spec"]):
+        assert np.array_equal(G.kgrid_map(int(g[0]), int(g[1]), *g[2:]), ref["grid_map_%d" % i]), i
+    for i, (ns, ends) in enumerate(zip(ref["grid_cut_ns"], ref["grid_cut_ends"])):
+        assert np.array_equal(G.kgrid_cut(int(ns), ends[0], ends[1]), ref["grid_cut_%d" % i]), i
+    for i, g in enumerate(ref["grid_log_spec"]):
+        assert np.array_equal(G.kgrid_logcut(int(g[0]), *g[1:]), ref["grid_log_%d" % i]), i
+    with pytest.raises(G.MsgpuError):
+        G.kgrid_logcut(5, 3.0, 0.1, -1.0)
