@@ -24,7 +24,18 @@ void launch_panel_rm(cx<T>* A, int np, long long mstride, int* ipiv, int* info, 
 // other columns, U12 of the columns right of it is solved, and those columns get the
 // rank-S update.  The pivots are recorded and the row maps built from them afterwards,
 // one lane per touched row.
-template <class T, int S, int R, int PT, bool RM>
+//
+// LL (left-looking, Crout order within the panel): a sub-panel receives all updates of
+// the panel's earlier sub-panels when it is loaded instead of after each of them, so the
+// columns right of a sub-panel are only read once and written once (their row
+// interchanges aside).  For the sub-panel [c0, c0+S) with K = c0 - k0 factored columns:
+//   U(k0:c0, c0:c0+S) = L(k0:c0, k0:c0)^-1 A(k0:c0, c0:c0+S)    (unit lower, staged in LDS;
+//        one lane per row, the solved rows broadcast by readlane)
+//   A(c0:np, c0:c0+S) -= L(c0:np, k0:c0) U(k0:c0, c0:c0+S)       (L read from the stored,
+//        already interchanged columns; U broadcast from LDS)
+// then the column loop as in the right-looking form.  The panel moves about 2.8 times its
+// algorithmic bytes with S = 8 instead of about 4.5.
+template <class T, int S, int R, int PT, bool RM, bool LL>
 __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int np, long long mstride,
                                                       int* __restrict__ ipiv_all, int* __restrict__ info_all,
                                                       int* __restrict__ pmap_all, int k0, int nb, int xa, int xb) {
@@ -35,7 +46,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
     __shared__ cx<T> s_row[2][PW][S];
     __shared__ cx<T> s_rowj[2][S];
     __shared__ cx<T> s_L[S][S];
-    __shared__ cx<T> s_U[S][64];
+    __shared__ cx<T> s_U[LL ? 1 : S][64];
     __shared__ int s_piv[64];
     __shared__ int s_sd[2 * S], s_ss[2 * S], s_scnt;
     __shared__ int s_pd[128], s_ps[128], s_pcnt;        // the panel's composite map, for columns [xa, xb)
@@ -43,8 +54,12 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
     // time, whenever they fit beside the arrays above within the 160 KB of LDS.  The
     // 256-thread shapes then need 45 KB or less and half the register file, which leaves
     // room on the CU for the GEMM workgroups of other streams.
-    constexpr bool LDSU = (size_t)PT * S * sizeof(cx<T>) <= 139264;
+    constexpr bool LDSU = !LL && (size_t)PT * S * sizeof(cx<T>) <= 139264;
     __shared__ cx<T> sL[LDSU ? S : 1][LDSU ? PT : 1];
+    // LL: the strictly lower part of the panel's factored top rows (packed, row r at
+    // r (r - 1) / 2) and the current sub-panel's top rows
+    __shared__ cx<T> sLt[LL ? 64 * 63 / 2 : 1];
+    __shared__ cx<T> sUt[LL ? 64 : 1][S];
 
     const int b = blockIdx.x;
     cx<T>* M = A + (size_t)b * mstride;
@@ -55,6 +70,35 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
     const int kend = min(k0 + nb, np);
 
     for (int c0 = k0; c0 < kend; c0 += S) {
+        const int K = c0 - k0;
+        if constexpr (LL) {
+            if (K > 0) {
+                // stage the unit-lower L of rows [k0, c0) and the sub-panel's top rows
+                for (int e = tid; e < K * K; e += PT) {
+                    const int i = RM ? e / K : e % K, m = RM ? e % K : e / K;
+                    if (i > m) sLt[(i * (i - 1) >> 1) + m] = M[at<RM>(k0 + i, k0 + m, np)];
+                }
+                for (int e = tid; e < K * S; e += PT) {
+                    const int i = RM ? e / S : e % K, c = RM ? e % S : e / K;
+                    sUt[i][c] = M[at<RM>(k0 + i, c0 + c, np)];
+                }
+                __syncthreads();
+                // U(k0:c0, c) by forward substitution: lane i holds row i; row m, once
+                // final, reaches the rows below it by readlane
+                for (int c = wv; c < S; c += PW) {
+                    cx<T> u = lane < K ? sUt[lane][c] : cx<T>{T(0), T(0)};
+                    for (int m = 0; m + 1 < K; m++) {
+                        const cx<T> um{rdlane(u.x, m), rdlane(u.y, m)};
+                        if (lane > m && lane < K) u = u - sLt[(lane * (lane - 1) >> 1) + m] * um;
+                    }
+                    if (lane < K) {
+                        sUt[lane][c] = u;
+                        M[at<RM>(k0 + lane, c0 + c, np)] = u;
+                    }
+                }
+                __syncthreads();
+            }
+        }
         cx<T> x[R][S];
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -62,6 +106,52 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
 #pragma unroll
             for (int c = 0; c < S; c++)
                 x[r][c] = (g < np) ? M[at<RM>(g, c0 + c, np)] : cx<T>{T(0), T(0)};
+        }
+        if constexpr (LL && RM) {
+            // rows [c0, np): A(g, c0:c0+S) -= L(g, k0:c0) U(k0:c0, c0:c0+S).  Row-major: a
+            // thread reads eight consecutive entries of its row (one 128-byte line) at a
+            // time, so every line fetched is used whole
+            for (int m0 = 0; m0 < K; m0 += 8) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int g = c0 + tid + PT * r;
+                    if (g >= np) continue;
+                    cx<T> l[8];
+#pragma unroll
+                    for (int h = 0; h < 8; h++)
+                        l[h] = (m0 + h < K) ? M[at<true>(g, k0 + m0 + h, np)] : cx<T>{T(0), T(0)};
+#pragma unroll
+                    for (int h = 0; h < 8; h++) {
+                        if (m0 + h >= K) break;
+#pragma unroll
+                        for (int c = 0; c < S; c++) x[r][c] = x[r][c] - l[h] * sUt[m0 + h][c];
+                    }
+                }
+            }
+        }
+        if constexpr (LL && !RM) {
+            // rows [c0, np): A(g, c0:c0+S) -= L(g, k0:c0) U(k0:c0, c0:c0+S).  Column-major:
+            // the threads' rows are consecutive in memory, two L columns per round in flight
+            for (int m = 0; m < K; m += 2) {
+                cx<T> l[R][2];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int g = c0 + tid + PT * r;
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+                        l[r][h] = (g < np && m + h < K) ? M[at<RM>(g, k0 + m + h, np)] : cx<T>{T(0), T(0)};
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (m + h >= K) break;
+#pragma unroll
+                    for (int c = 0; c < S; c++) {
+                        const cx<T> u = sUt[m + h][c];
+#pragma unroll
+                        for (int r = 0; r < R; r++) x[r][c] = x[r][c] - l[r][h] * u;
+                    }
+                }
+            }
         }
         // Column loop with a rotating register file: at step jj, x[r][c] holds column
         // c0 + (jj + c) % S, so the active column is always x[r][0] and every register
@@ -217,7 +307,7 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
             }
             __syncthreads();
         }
-        if (nright > 0) {
+        if (!LL && nright > 0) {
             // U12' = L11'^-1 A12' for the panel columns right of the sub-panel
             if (tid < nright) {          // solved in LDS: the registers hold the sub-panel
                 const int col = c0 + S + tid;
@@ -395,55 +485,67 @@ __global__ __launch_bounds__(PT) void lu_panel_kernel(cx<T>* __restrict__ A, int
 // ------------------------------------------------------------------ launchers
 // Panel shapes (threads PT, sub-panel width S, register rows R = rows / PT), chosen
 // per precision and padded size from measurements of the panel microbenchmark.
-template <class T, bool RM, int S, int R, int PT>
+template <class T, bool RM, int S, int R, int PT, bool LL = false>
 static void panel_go(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb,
                      int nbatch, hipStream_t st, int xa, int xb) {
-    hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, RM>), dim3(nbatch), dim3(PT), 0, st, A, np, mstride, ipiv,
-                       info, pmap, k0, nb, xa, xb);
+    hipLaunchKernelGGL((lu_panel_kernel<T, S, R, PT, RM, LL>), dim3(nbatch), dim3(PT), 0, st, A, np, mstride,
+                       ipiv, info, pmap, k0, nb, xa, xb);
 }
 
 // Panel shapes <S, R, PT> per size range, tried by the autotuner; variant 0 is the
 // default.  PT x R must cover np rows.  The 1024-thread shapes are limited to 128
 // VGPRs per lane; the 512-thread ones may use 256.
-template <class T, bool RM>
-void launch_panel_rm(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb,
+template <class T, bool RM, bool LL>
+static void panel_dispatch(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb,
                      int nbatch, hipStream_t st, int variant, int xa, int xb) {
     constexpr bool D = sizeof(T) == 8;
-    if (variant < 0 || variant >= kPanelVariants) variant = 0;
     if (variant == 4) {          // 16-column sub-panels: half the passes over the panel's right part
         if (np <= 1024) {
-            if (D) panel_go<T, RM, 16, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-            else panel_go<T, RM, 16, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+            if (D) panel_go<T, RM, 16, 2, 512, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+            else panel_go<T, RM, 16, 1, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
             return;
         }
         variant = 0;
     }
     if (variant == 3) {          // 256 threads: leaves room on the CU for GEMM waves of another stream
-        if (np <= 1024) panel_go<T, RM, 8, 4, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else if (np <= 2048) panel_go<T, RM, 4, 8, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else if (np <= 4096) panel_go<T, RM, 2, 16, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else panel_go<T, RM, 1, 32, 256>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        if (np <= 1024) panel_go<T, RM, 8, 4, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else if (np <= 2048) panel_go<T, RM, 4, 8, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else if (np <= 4096) panel_go<T, RM, 2, 16, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else panel_go<T, RM, 1, 32, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
         return;
     }
     // up to N = 2048 the defaults are the 1024-thread shapes whose rank-S update reads the
     // L rows from LDS (at N = 1024-2048 they take 0.4-0.7 of the 512-thread shapes' time)
     if (np <= 1024) {
-        if (variant == 1) panel_go<T, RM, 8, 2, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else if (variant == 2) panel_go<T, RM, 4, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else panel_go<T, RM, 8, 1, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        if (variant == 1) panel_go<T, RM, 8, 2, 512, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else if (variant == 2) panel_go<T, RM, 4, 1, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else panel_go<T, RM, 8, 1, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
     } else if (np <= 2048) {
-        if (variant == 2) panel_go<T, RM, 8, 4, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else if ((variant == 0) == D) panel_go<T, RM, 4, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else panel_go<T, RM, 8, 2, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        if (variant == 2) panel_go<T, RM, 8, 4, 512, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else if ((variant == 0) == D) panel_go<T, RM, 4, 2, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else panel_go<T, RM, 8, 2, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
     } else if (np <= 4096) {     // complex128 default: the 512-thread shape, which does not spill
-        if (variant == 1) panel_go<T, RM, 2, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else if ((variant == 0) == D) panel_go<T, RM, 4, 8, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else panel_go<T, RM, 4, 4, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        if (variant == 1) panel_go<T, RM, 2, 4, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else if ((variant == 0) == D) panel_go<T, RM, 4, 8, 512, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else panel_go<T, RM, 4, 4, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
     } else {
-        if (variant == 1) panel_go<T, RM, 1, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else if ((variant == 0) == D) panel_go<T, RM, 2, 16, 512>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
-        else panel_go<T, RM, 2, 8, 1024>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        if (variant == 1) panel_go<T, RM, 1, 8, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else if ((variant == 0) == D) panel_go<T, RM, 2, 16, 512, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else panel_go<T, RM, 2, 8, 1024, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
     }
+}
+
+// Variants 0-4: right-looking shapes; 5, 6, 7: the left-looking form (LL) of variants 3,
+// 0 and 4.
+template <class T, bool RM>
+void launch_panel_rm(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb,
+                     int nbatch, hipStream_t st, int variant, int xa, int xb) {
+    if (variant < 0 || variant >= kPanelVariants) variant = 0;
+    if (variant >= 5)
+        panel_dispatch<T, RM, true>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, variant == 5 ? 3 : variant == 6 ? 0 : 4,
+                                    xa, xb);
+    else
+        panel_dispatch<T, RM, false>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, variant, xa, xb);
 }
 
 #endif  // MSG_PANEL_IMPL

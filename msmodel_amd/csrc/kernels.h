@@ -39,8 +39,9 @@ void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double
 
 // kern_lu.hip
 // rm: the matrices are stored row-major (see at<> in kern_lu.hip); variant: panel
-// shape index < kPanelVariants (0 = default for the size range)
-constexpr int kPanelVariants = 5;
+// shape index < kPanelVariants (0 = default for the size range; 5-7 the left-looking forms of
+// shapes 3, 0 and 4)
+constexpr int kPanelVariants = 8;
 template <class T>
 void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int* pmap, int k0, int nb, int nbatch,
                   hipStream_t st, bool rm = false, int variant = 0, int xa = 0, int xb = 0);

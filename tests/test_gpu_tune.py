@@ -56,7 +56,8 @@ def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
     ref = None
     for v in range(6):
         for chunk, streams, rm, pv, b2 in ((256, 4, 0, 0, 0), (16, 1, 0, 1, 1), (256, 2, 1, 2, 0), (16, 1, 1, v % 5, 1),
-                                           (128, 2, 0, 3, 1), (64, 1, 0, 4, 0)):
+                                           (128, 2, 0, 3, 1), (64, 1, 0, 4, 0), (32, 2, 1, 5 + v % 3, 1),
+                                           (32, 1, 0, 5 + (v + 1) % 3, 0)):
             path = os.path.join(str(tmp_path), "tune_%d_%d_%d_%d.txt" % (v, chunk, rm, b2))
             with open(path, "w") as f:
                 f.write("384 %d %d %d %d %d %d 0 %d\n" % (prec, v, chunk, streams, rm, pv, b2))
@@ -64,9 +65,10 @@ def test_every_swap_variant_same_result(G, tmp_path, prec_mode):
             out, _ = G.tracelog_batch(med, ks, mode)
             if ref is None:
                 ref = out
-            if rm == 0 and b2 == 0:     # launch shapes only: exact
+            if rm == 0 and b2 == 0 and pv < 5:     # launch shapes only: exact
                 assert np.array_equal(out, ref), (v, chunk, streams, b2)
-            else:   # row-major storage and the 128-column blocking (3M updates) round differently
+            else:   # row-major storage, the 128-column blocking (3M updates) and the left-looking
+                    # panels (another summation order) round differently
                 tol = 1e-10 if prec == 0 else 1e-4
                 assert np.abs(out.real - ref.real).max() <= tol * max(1, np.abs(ref.real).max())
                 dim = (330 * (out.imag - ref.imag) + np.pi) % (2 * np.pi) - np.pi
@@ -105,11 +107,13 @@ def test_graph_replays_match_eager(G, tmp_path, prec_mode):
 
 
 @pytest.mark.parametrize("prec_mode", ["FP64", "FP32"])
-@pytest.mark.parametrize("n,np_,variants", [(1000, 1024, (0, 1, 2, 3, 4)), (1100, 1152, (0, 1, 2, 3))])
+@pytest.mark.parametrize("n,np_,variants", [(1000, 1024, (0, 1, 2, 3, 4, 5, 6, 7)), (1100, 1152, (0, 1, 2, 3, 5, 6)),
+                                             (2000, 2048, (0, 3, 5, 6))])
 def test_panel_shapes_beyond_one_register_row(G, tmp_path, prec_mode, n, np_, variants):
     """Panel shapes whose threads hold several rows of the panel (Np above the thread
-    count: the rank-S update stages one register row at a time): every shape gives the
-    same trace-log, row-major with 128-column blocks, and one cell matches the oracle."""
+    count: the rank-S update stages one register row at a time), right-looking (0-4) and
+    left-looking (5-7): every shape gives the same trace-log, row-major with 128-column
+    blocks, and one cell matches the oracle."""
     mode = getattr(G, prec_mode)
     prec = 0 if mode == G.FP64 else 1
     pos = O.fill_medium(3, n, 12)
