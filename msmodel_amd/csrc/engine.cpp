@@ -536,32 +536,38 @@ static void lu_solve(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, co
 // Y = P V, i.e. Y[perm[i]] = V[i] with perm from launch_build_perm.
 // Blocks of 128 rows (two 64-row triangular blocks and the 64-row update between them),
 // so that the updates of the rest of X have K = 128 (one pass over X per 128 rows).
+// identity_rhs: X is the identity (np columns), whose column c is zero above row c: the
+// forward solve then leaves every column c >= the end of the current block untouched,
+// so each forward step runs on the columns [0, block end) only (a third of the forward
+// work; the inverse then costs 16 N^3 / 3 real flops instead of 8 N^3).
 template <class T>
-static void lu_solve_rm(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, long long sx, hipStream_t st) {
+static void lu_solve_rm(const cx<T>* lu, cx<T>* X, int np, int ncol, int nbatch, long long sx, hipStream_t st,
+                        bool identity_rhs = false) {
     const long long ms = (long long)np * np;
-    // X[r0:r0+m] -= S[r0:r0+m, c0:c0+k] X[c0:c0+k]
-    auto upd = [&](int r0, int c0, int m, int k) {
-        if (m > 0 && k > 0)
-            launch_gemm_sub<T>(X + r0, lu + r0 + (size_t)c0 * np, X + c0, m, ncol, k, np, np, np, sx, ms, sx, nbatch,
+    // X[r0:r0+m, 0:n] -= S[r0:r0+m, c0:c0+k] X[c0:c0+k, 0:n]
+    auto upd = [&](int r0, int c0, int m, int k, int n) {
+        if (m > 0 && k > 0 && n > 0)
+            launch_gemm_sub<T>(X + r0, lu + r0 + (size_t)c0 * np, X + c0, m, n, k, np, np, np, sx, ms, sx, nbatch,
                                st);
     };
+    auto fwd_cols = [&](int end) { return identity_rhs ? std::min(ncol, end) : ncol; };
     for (int K0 = 0; K0 < np; K0 += 2 * NB) {
         const int k1 = K0 + NB, ke = std::min(K0 + 2 * NB, np);
-        launch_trsm_block<T>(lu, X, np, ncol, K0, 0, ms, sx, nbatch, st, true);
+        launch_trsm_block<T>(lu, X, np, fwd_cols(k1), K0, 0, ms, sx, nbatch, st, true);
         if (k1 < np) {
-            upd(k1, K0, ke - k1, NB);
-            launch_trsm_block<T>(lu, X, np, ncol, k1, 0, ms, sx, nbatch, st, true);
+            upd(k1, K0, ke - k1, NB, fwd_cols(k1));
+            launch_trsm_block<T>(lu, X, np, fwd_cols(ke), k1, 0, ms, sx, nbatch, st, true);
         }
-        upd(ke, K0, np - ke, ke - K0);
+        upd(ke, K0, np - ke, ke - K0, fwd_cols(ke));
     }
     for (int K0 = ((np - 1) / (2 * NB)) * (2 * NB); K0 >= 0; K0 -= 2 * NB) {
         const int k1 = K0 + NB, ke = std::min(K0 + 2 * NB, np);
         if (k1 < np) {
             launch_trsm_block<T>(lu, X, np, ncol, k1, 1, ms, sx, nbatch, st, true);
-            upd(K0, k1, NB, ke - k1);
+            upd(K0, k1, NB, ke - k1, ncol);
         }
         launch_trsm_block<T>(lu, X, np, ncol, K0, 1, ms, sx, nbatch, st, true);
-        upd(0, K0, K0, ke - K0);
+        upd(0, K0, K0, ke - K0, ncol);
     }
     HIPCHK(hipGetLastError());
 }
@@ -890,6 +896,10 @@ static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx
     const int nlane = std::min<int>(S.ns, (int)P.chunks.size());
     std::vector<DBuf<cx<T>>> mats(nlane), rhs(nlane);
     std::vector<DBuf<int>> ipiv(nlane), info(nlane), pmap(nlane), perm(nlane);
+    std::vector<DBuf<cx<double>>> partial(nlane);
+    std::vector<DBuf<int>> invp(nlane);
+    size_t nparts = 1;          // partial sums of the contraction per chunk
+    for (const Plan::Chunk& ch : P.chunks) nparts = std::max(nparts, (size_t)ch.count * dk_contract_partials(ch.np));
     for (int i = 0; i < nlane; i++) {
         mats[i].alloc(P.mat_elems);
         rhs[i].alloc(P.mat_elems);
@@ -897,6 +907,8 @@ static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx
         perm[i].alloc(P.piv_ints);
         info[i].alloc(P.max_b);
         pmap[i].alloc(P.map_ints);
+        partial[i].alloc(nparts);
+        invp[i].alloc(P.piv_ints);
     }
     for (size_t c = 0; c < P.chunks.size(); c++) {
         const Plan::Chunk& ch = P.chunks[c];
@@ -904,14 +916,17 @@ static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx
         const long long ms = (long long)np * np;
         hipStream_t st = S[ln];
         const MatItem* it = D.items.p + ch.start;
+        // M^-1 = P V with V the solve of the row-major factor against the identity; then
+        //   tr(M^-1)    = sum_i V[i, perm[i]]                 (traceinv)
+        //   tr(M^-1 M') = sum_i sum_c V[i, c] M'[perm[i], c]  (M' generated on the fly)
         launch_assemble_items<T>(it, np, 0, mats[ln].p, ms, B, st);
-        if (trace_inv) HIPCHK(launch_set_identity<T>(rhs[ln].p, np, ms, B, st));    // tr(M^-1)
-        else launch_assemble_items<T>(it, np, 1, rhs[ln].p, ms, B, st);
-        // tr(M^-1 M') = tr(P V) = sum_i V[i, perm[i]] with V from lu_solve_rm
+        HIPCHK(launch_set_identity<T>(rhs[ln].p, np, ms, B, st));
         factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, true, true, st);
-        lu_solve_rm<T>(mats[ln].p, rhs[ln].p, np, np, B, ms, st);
+        lu_solve_rm<T>(mats[ln].p, rhs[ln].p, np, np, B, ms, st, true);
         launch_build_perm(pmap[ln].p, np, NB, perm[ln].p, B, st);
-        launch_diag_sum<T>(rhs[ln].p, 0, it, np, ms, dout.p + ch.start, B, st, perm[ln].p);
+        if (trace_inv) launch_diag_sum<T>(rhs[ln].p, 0, it, np, ms, dout.p + ch.start, B, st, perm[ln].p);
+        else launch_dk_contract<T>(rhs[ln].p, ms, np, it, perm[ln].p, invp[ln].p, partial[ln].p, dout.p + ch.start,
+                                   B, st);
     }
     S.join();
     std::vector<cx<double>> hout(nk);

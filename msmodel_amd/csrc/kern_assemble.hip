@@ -158,6 +158,125 @@ template void launch_assemble<double>(const double*, int, int, int, const cx<dou
 template void launch_assemble<float>(const double*, int, int, int, const cx<double>*, const cx<double>*,
                                      const cx<double>*, int, cx<float>*, long long, int, hipStream_t);
 
+// ------------------------------------------------------------------ tr(M^-1 M') by contraction
+// For each matrix of a batch, from V = the solution of the row-major factorisation
+// against the identity (M^-1 = P V: row a of M^-1 is row inv[a] of V, inv the inverse
+// of the row permutation; column-major, ld np):
+//   tr(M^-1 M') = sum_{a,c} (M^-1)_{ac} M'_{ca}
+//              = sum_a (M^-1)_{aa} M'_{aa} + sum_{a<c} M'_{ac} ((M^-1)_{ac} + (M^-1)_{ca})
+// since M' = dM/dk is symmetric.  M' is generated pair by pair as build_dk_ms_matrix does
+// (ms_matrix_util.c:68-81): -G'_d off the diagonal, the finite-difference derivative of
+// F^-1 on it; no M' matrix is stored.  One workgroup per pair of 32 x 32 atom tiles
+// (I <= J) of one matrix, as the assembly: one Green evaluation per unordered pair; the
+// two V blocks (rows inv[I], columns J and rows inv[J], columns I) are read column by
+// column (coalesced), the second through an LDS transpose.  Each workgroup writes one
+// FP64 partial sum; dk_reduce_kernel adds a matrix's partial sums in a fixed order.
+constexpr int DKT = 32;
+
+template <class T>
+__global__ __launch_bounds__(256) void dk_contract_kernel(const cx<T>* __restrict__ V, long long sv, int np,
+                                                          const MatItem* __restrict__ items,
+                                                          const int* __restrict__ inv_all,
+                                                          cx<double>* __restrict__ partial) {
+    __shared__ double sx[2][DKT][4];
+    __shared__ int sinv[2][DKT];
+    __shared__ cx<T> vt[DKT][DKT + 1];       // V(inv[J], I) transposed: vt[a - I*DKT][c - J*DKT]
+    __shared__ double sre[256], sim[256];
+    const int b = blockIdx.y, tid = threadIdx.x;
+    const MatItem& it = items[b];
+    const int n = it.n, d = it.d;
+    const int nt = np / DKT;
+    int t = blockIdx.x, I = 0;
+    while (t >= nt - I) { t -= nt - I; I++; }
+    const int J = I + t;
+    const cx<T>* Vb = V + (size_t)b * sv;
+    const int* inv = inv_all + (size_t)b * np;
+    if (tid < 2 * DKT) {
+        const int side = tid / DKT, q = tid % DKT;
+        const int atom = (side == 0 ? I : J) * DKT + q;
+        for (int c = 0; c < 4; c++) sx[side][q][c] = (atom < n && c < d) ? it.pos[(size_t)atom * d + c] : 0.0;
+        sinv[side][q] = atom < n ? inv[atom] : 0;
+    }
+    __syncthreads();
+    const int ii = tid % DKT, j0 = tid / DKT;          // thread: row ii of the tile, columns j0 + 8 q
+    // V(inv[c], a) for c in tile J (rows), a in tile I (columns), stored transposed
+#pragma unroll
+    for (int q = 0; q < DKT / 8; q++) {
+        const int jj = j0 + 8 * q;
+        const int c = J * DKT + ii, a = I * DKT + jj;
+        vt[jj][ii] = (c < n && a < n) ? Vb[(size_t)sinv[1][ii] + (size_t)a * np] : cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    const cx<T> k{(T)it.k.x, (T)it.k.y};
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int q = 0; q < DKT / 8; q++) {
+        const int jj = j0 + 8 * q;
+        const int a = I * DKT + ii, c = J * DKT + jj;
+        if (a >= n || c >= n || (I == J && a > c)) continue;
+        const cx<T> v1 = Vb[(size_t)sinv[0][ii] + (size_t)c * np];      // (M^-1)_{ac}
+        cx<T> w, g;
+        if (a == c) {
+            w = v1;
+            g = cx<T>{(T)it.diag1.x, (T)it.diag1.y};
+        } else {
+            w = v1 + vt[ii][jj];                                           // + (M^-1)_{ca}
+            double s = 0.0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const double dd = sx[0][ii][e] - sx[1][jj][e];
+                s += dd * dd;
+            }
+            g = neg_dk_green<T>(d, k, (T)sqrt(s));
+        }
+        re += (double)w.x * (double)g.x - (double)w.y * (double)g.y;
+        im += (double)w.x * (double)g.y + (double)w.y * (double)g.x;
+    }
+    sre[tid] = re;
+    sim[tid] = im;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) { sre[tid] += sre[tid + h]; sim[tid] += sim[tid + h]; }
+        __syncthreads();
+    }
+    if (tid == 0) partial[(size_t)b * gridDim.x + blockIdx.x] = cx<double>{sre[0], sim[0]};
+}
+
+__global__ void dk_reduce_kernel(const cx<double>* __restrict__ partial, int nblk, int nbatch,
+                                 cx<double>* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbatch) return;
+    cx<double> s{0.0, 0.0};
+    for (int j = 0; j < nblk; j++) s = s + partial[(size_t)b * nblk + j];
+    out[b] = s;
+}
+
+// inv[b][perm[b][i]] = i
+__global__ void invert_perm_kernel(const int* __restrict__ perm, int np, int* __restrict__ inv) {
+    const int b = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) inv[(size_t)b * np + perm[(size_t)b * np + i]] = i;
+}
+
+size_t dk_contract_partials(int np) {
+    const size_t nt = (size_t)np / DKT;
+    return nt * (nt + 1) / 2;
+}
+
+template <class T>
+void launch_dk_contract(const cx<T>* V, long long sv, int np, const MatItem* items, const int* perm, int* inv,
+                        cx<double>* partial, cx<double>* out, int nbatch, hipStream_t st) {
+    if (nbatch <= 0) return;
+    hipLaunchKernelGGL(invert_perm_kernel, dim3((np + 255) / 256, nbatch), dim3(256), 0, st, perm, np, inv);
+    const int nblk = (int)dk_contract_partials(np);
+    hipLaunchKernelGGL(dk_contract_kernel<T>, dim3(nblk, nbatch), dim3(256), 0, st, V, sv, np, items, inv,
+                       partial);
+    hipLaunchKernelGGL(dk_reduce_kernel, dim3((nbatch + 63) / 64), dim3(64), 0, st, partial, nblk, nbatch, out);
+}
+template void launch_dk_contract<double>(const cx<double>*, long long, int, const MatItem*, const int*, int*,
+                                         cx<double>*, cx<double>*, int, hipStream_t);
+template void launch_dk_contract<float>(const cx<float>*, long long, int, const MatItem*, const int*, int*,
+                                        cx<double>*, cx<double>*, int, hipStream_t);
+
 // ------------------------------------------------------------------ incident vectors
 // source 0: plane wave e^{ik x_0} (incident_wave_util.c:76); 1: spherical G_d(k,|x|);
 // 2: build_plane_wave with angle theta (ms_matrix_util.c:88-106).  Padding rows get 0.

@@ -26,6 +26,14 @@ void launch_plane_items(const MatItem* items, int np, cx<double>* phi, int nbatc
 template <class T>
 void launch_assemble_items(const MatItem* items, int np, int kind, cx<T>* out, long long mstride, int nbatch,
                            hipStream_t st);
+// out[b] = tr(M_b^-1 M_b') from V_b, the row-major factor's solve against the identity
+// (M^-1 = P V, perm from launch_build_perm), with M' generated on the fly; inv (np ints
+// per matrix) receives the inverse permutation, partial dk_contract_partials(np) sums per
+// matrix
+size_t dk_contract_partials(int np);
+template <class T>
+void launch_dk_contract(const cx<T>* V, long long sv, int np, const MatItem* items, const int* perm, int* inv,
+                        cx<double>* partial, cx<double>* out, int nbatch, hipStream_t st);
 void launch_diag(const cx<double>* ks, int nk, ScModel m, int d, int kind, cx<double>* diag, cx<double>* izero,
                  hipStream_t st);
 template <class T>
