@@ -193,8 +193,13 @@ def serving_stream(G, nreq, rate, nmin, nmax, seed=5):
     th.start()
     th.join()
     lat = {0: [], 1: [], 2: []}
-    for j, (kind, _, _, _) in enumerate(specs):
+    nbig = {0: 0, 1: 0, 2: 0}
+    for j, (kind, med, _, _) in enumerate(specs):
         lat[kind].append(srv.wait(ids[j])[1]["total_ms"])
+        nbig[kind] += med.n >= 4000
+        if nmax > 2000:
+            progress("request %d/%d (kind %d, d=%d, N=%d) done after %.0f ms" % (j + 1, nreq, kind, med.c.d, med.n,
+                                                                              lat[kind][-1]))
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
@@ -206,6 +211,7 @@ def serving_stream(G, nreq, rate, nmin, nmax, seed=5):
     return {"requests": nreq, "arrival_rate_per_s": rate, "N_range": [nmin, nmax], "wall_s": wall,
             "throughput_req_per_s": nreq / wall, "latency_ms": pct(list(allv)),
             "by_kind_ms": {"logdet_tile64": pct(lat[0]), "solve_psi256": pct(lat[1]), "resonance32": pct(lat[2])},
+            "requests_with_N_ge_4000_by_kind": {"logdet_tile64": nbig[0], "solve_psi256": nbig[1], "resonance32": nbig[2]},
             "steps": st["steps"], "batched_matrices": st["batched_matrices"],
             "max_requests_per_step": st["max_requests_per_step"]}
 
