@@ -419,13 +419,13 @@ def main():
                           "launches": {c: v["launches"] for c, v in prof.items()}}))
         return
 
-    def timed_path(mode):
+    def timed_path(mode, rel=1e-6):
         for _ in range(args.warmup):
-            G.tracelog_batch(med, ks, mode, 1e-6)
+            G.tracelog_batch(med, ks, mode, rel)
         G.profile_begin()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            out, flags = G.tracelog_batch(med, ks, mode, 1e-6)
+            out, flags = G.tracelog_batch(med, ks, mode, rel)
         dt = time.perf_counter() - t0
         return out, flags, dt, G.profile_end()
 
@@ -449,10 +449,28 @@ def main():
             G.tracelog_batch(med, np.array([k]), G.FP64)
         fp64_plain = len(sub) / (time.perf_counter() - t0)
     # arg det = N Im(tracelog) is defined modulo 2 pi: compare Im wrapped
-    dre = np.abs(out_mixed.real - out64.real)
-    dim = np.abs((n * (out_mixed.imag - out64.imag) + np.pi) % (2 * np.pi) - np.pi) / n
-    dtl = np.hypot(dre, dim)
+    def dtracelog(out):
+        dre = np.abs(out.real - out64.real)
+        dim = np.abs((n * (out.imag - out64.imag) + np.pi) % (2 * np.pi) - np.pi) / n
+        return np.hypot(dre, dim), dre
+
+    dtl, dre = dtracelog(out_mixed)
     dtl_re = float(dre.max())
+    # the pivot-growth gate (complex64 factor; complex128 where 16 eps32 exp(max log|U_ii| -
+    # min log|U_ii|) > tol) at the 1e-8 target and at a looser tolerance
+    gated = None
+    if not args.no_mixed:
+        gated = {}
+        for tol in (1e-8, 1e-5):
+            og, fg, tg, _ = timed_path(G.MIXED_GATED, tol)
+            e, _ = dtracelog(og)
+            gated["tol_%g" % tol] = {
+                "evals_per_s": args.steps * nk / tg, "recomputed_cells": int(fg.sum()), "cells": nk,
+                "max_abs_dtracelog": float(e.max()), "median_abs_dtracelog": float(np.median(e)),
+                "p99_abs_dtracelog": float(np.percentile(e, 99)), "cells_within_tol": int((e <= tol).sum()),
+                "max_abs_dtracelog_kept_cells": float(e[fg == 0].max()) if (fg == 0).any() else 0.0}
+            progress("gated mixed map, tol %g: %.1f evals/s, %d cells recomputed" % (tol, args.steps * nk / tg,
+                                                                                   int(fg.sum())))
 
     serving = uneven = wave = small = None
     if args.requests > 0:
@@ -532,6 +550,7 @@ def main():
         "mixed_p90_p99_abs_dtracelog": [float(np.percentile(dtl, 90)), float(np.percentile(dtl, 99))],
         "mixed_cells_within_1e-8": int((dtl <= 1e-8).sum()),
         "mixed_max_abs_dtracelog_re": dtl_re, "roofline_c64_gemm": roofline_c64,
+        "mixed_gated": gated,
         "resonance_latency_ms": serving["by_kind_ms"]["resonance32"] if serving else None,
         "serving": serving,
         "uneven_resonance_search": uneven,

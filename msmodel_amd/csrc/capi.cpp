@@ -70,24 +70,33 @@ int tracelog_any(const std::vector<MediumView>& mv, int nk, const int* midx, con
         if (flags && nk > 0) memset(flags, 0, sizeof(int) * nk);
         return tracelog_multi(mv, nk, midx, as_cx(ks), as_cx(out), nullptr, mode == MSGPU_FP32 ? 1 : 0);
     }
-    if (mode != MSGPU_MIXED) { set_error("msgpu: unknown mode"); return -4; }
+    if (mode != MSGPU_MIXED && mode != MSGPU_MIXED_GATED) { set_error("msgpu: unknown mode"); return -4; }
     if (nk <= 0) return 0;
-    int rc = tracelog_multi(mv, nk, midx, as_cx(ks), as_cx(out), nullptr, 1);
+    const bool gated = mode == MSGPU_MIXED_GATED;
+    std::vector<double> st(gated ? 2 * (size_t)nk : 0);
+    int rc = tracelog_multi(mv, nk, midx, as_cx(ks), as_cx(out), gated ? st.data() : nullptr, 1);
     if (rc < 0) return rc;
-    // per medium, the cells of that medium's items to recompute
-    std::vector<std::vector<int>> per(mv.size());
-    for (int l = 0; l < nk; l++) per[midx ? midx[l] : 0].push_back(l);
     std::vector<int> idx, sub_m;
     if (flags) memset(flags, 0, sizeof(int) * nk);
-    for (size_t m = 0; m < mv.size(); m++) {
-        if (per[m].empty()) continue;
-        std::vector<msgpu_cplx> t(per[m].size());
-        for (size_t q = 0; q < per[m].size(); q++) t[q] = out[per[m][q]];
-        for (int q : reeval_cells(mv[m].n, (int)t.size(), t.data(), reeval_rel)) {
-            const int l = per[m][q];
+    if (gated) {        // the pivot-growth gate, cell by cell
+        for (int l : gated_cells(nk, st.data(), out, reeval_rel)) {
             if (flags) flags[l] = 1;
             idx.push_back(l);
-            sub_m.push_back((int)m);
+            sub_m.push_back(midx ? midx[l] : 0);
+        }
+    } else {            // per medium, the cells below reeval_rel of its largest |det|
+        std::vector<std::vector<int>> per(mv.size());
+        for (int l = 0; l < nk; l++) per[midx ? midx[l] : 0].push_back(l);
+        for (size_t m = 0; m < mv.size(); m++) {
+            if (per[m].empty()) continue;
+            std::vector<msgpu_cplx> t(per[m].size());
+            for (size_t q = 0; q < per[m].size(); q++) t[q] = out[per[m][q]];
+            for (int q : reeval_cells(mv[m].n, (int)t.size(), t.data(), reeval_rel)) {
+                const int l = per[m][q];
+                if (flags) flags[l] = 1;
+                idx.push_back(l);
+                sub_m.push_back((int)m);
+            }
         }
     }
     if (!idx.empty()) {

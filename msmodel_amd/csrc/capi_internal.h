@@ -29,6 +29,21 @@ inline std::vector<int> reeval_cells(int n, int nk, const std::complex<double>* 
     return idx;
 }
 
+// Cells of a complex64 trace-log map to recompute in complex128 under the pivot-growth
+// gate: the error bound 16 eps32 exp(max log|U_ii| - min log|U_ii|) of the cell's complex64
+// factor (stats[2l], stats[2l+1]) exceeds tol (<= 0: 1e-8), or the value is not finite.
+constexpr double kGateFactor = 16.0;
+inline double gate_bound(const double* st) { return kGateFactor * 5.9604644775390625e-08 * std::exp(st[0] - st[1]); }
+inline std::vector<int> gated_cells(int nk, const double* stats, const std::complex<double>* t, double tol) {
+    if (tol <= 0) tol = 1e-8;
+    std::vector<int> idx;
+    for (int l = 0; l < nk; l++) {
+        const double b = gate_bound(stats + 2 * l);
+        if (!(b <= tol) || !std::isfinite(t[l].real()) || !std::isfinite(t[l].imag())) idx.push_back(l);
+    }
+    return idx;
+}
+
 // medium.cpp
 double ball_volume(int d);
 double ball_radius(int d, int n);

@@ -55,6 +55,7 @@ struct Req {
     const double* dpos = nullptr;       // resident device copy while admitted
     size_t dtok = 0;
     std::vector<cd> ks;                 // LOGDET: copy of the k points
+    std::vector<double> stats;          // LOGDET, MSGPU_MIXED_GATED: the complex64 pivot statistics
     int stage = 0;                      // LOGDET: 0 first pass, 1 FP64 re-evaluation
     std::vector<int> reidx;
     std::unique_ptr<AberthRun> run;     // RESONANCE
@@ -191,6 +192,7 @@ void Server::batch_loop() {
         // The step: (op, precision) -> items; op 0 trace-log, 1 tr(M^-1 M').
         struct Batch {
             std::vector<msgpu_cplx> z, f;
+            std::vector<double> st;          // pivot statistics (complex64 trace-logs)
             std::vector<int> m;
             std::vector<Req*> owner;
             std::vector<int> slot;
@@ -236,8 +238,9 @@ void Server::batch_loop() {
                 b.f.resize(b.z.size());
                 futs.push_back(std::async(std::launch::async, [&, op, p] {
                     Batch& bb = bt[op][p];
+                    if (op == 0 && p == 1) bb.st.resize(2 * bb.z.size());
                     rc[op][p] = op == 0 ? tracelog_multi(views, (int)bb.z.size(), bb.m.data(), as_cx(bb.z.data()),
-                                                         as_cx(bb.f.data()), nullptr, p)
+                                                         as_cx(bb.f.data()), p == 1 ? bb.st.data() : nullptr, p)
                                         : dk_log_det_multi(views, (int)bb.z.size(), bb.m.data(), as_cx(bb.z.data()),
                                                            as_cx(bb.f.data()), p);
                     if (rc[op][p] < 0) errs[op][p] = last_error();
@@ -253,8 +256,16 @@ void Server::batch_loop() {
                 for (size_t i = 0; i < b.z.size(); i++) {
                     Req* q = b.owner[i];
                     if (rc[op][p] < 0) { failed[q] = rc[op][p]; q->err = errs[op][p]; continue; }
-                    if (q->r.kind == MSGPU_REQ_LOGDET) q->r.out[b.slot[i]] = b.f[i];
-                    else fvals[q].push_back(b.f[i]);
+                    if (q->r.kind == MSGPU_REQ_LOGDET) {
+                        q->r.out[b.slot[i]] = b.f[i];
+                        if (op == 0 && p == 1 && q->r.mode == MSGPU_MIXED_GATED && q->stage == 0) {
+                            q->stats.resize(2 * q->ks.size());
+                            q->stats[2 * b.slot[i]] = b.st[2 * i];
+                            q->stats[2 * b.slot[i] + 1] = b.st[2 * i + 1];
+                        }
+                    } else {
+                        fvals[q].push_back(b.f[i]);
+                    }
                 }
             }
         std::lock_guard<std::mutex> g(mu);
@@ -267,8 +278,10 @@ void Server::batch_loop() {
                 q->status = failed[q];
                 fin = true;
             } else if (q->r.kind == MSGPU_REQ_LOGDET) {
-                if (q->stage == 0 && q->r.mode == MSGPU_MIXED) {
-                    q->reidx = reeval_cells(q->mv.n, q->r.nk, q->r.out, q->r.reeval_rel);
+                if (q->stage == 0 && (q->r.mode == MSGPU_MIXED || q->r.mode == MSGPU_MIXED_GATED)) {
+                    q->reidx = q->r.mode == MSGPU_MIXED
+                                   ? reeval_cells(q->mv.n, q->r.nk, q->r.out, q->r.reeval_rel)
+                                   : gated_cells(q->r.nk, q->stats.data(), q->r.out, q->r.reeval_rel);
                     q->count = (int)q->reidx.size();
                     q->stage = 1;
                     fin = q->reidx.empty();
@@ -319,8 +332,10 @@ MSGPU_API long long msgpu_server_submit(msgpu_server* sv, const msgpu_request* r
         set_error("msgpu_server_submit: the device path supports d = 1..4 and 1 <= N <= 8192");
         return -3;
     }
-    if (r->mode != MSGPU_FP64 && r->mode != MSGPU_MIXED && r->mode != MSGPU_FP32) {
-        set_error("msgpu_server_submit: unknown mode");
+    if ((r->mode != MSGPU_FP64 && r->mode != MSGPU_MIXED && r->mode != MSGPU_FP32 &&
+         r->mode != MSGPU_MIXED_GATED) ||
+        (r->mode == MSGPU_MIXED_GATED && r->kind != MSGPU_REQ_LOGDET)) {
+        set_error("msgpu_server_submit: unknown mode (MSGPU_MIXED_GATED is for log-det tiles)");
         return -4;
     }
     const bool ok = (r->kind == MSGPU_REQ_LOGDET && r->nk > 0 && r->ks && r->out) ||

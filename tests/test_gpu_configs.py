@@ -138,3 +138,31 @@ def test_uneven_resonance_search(G):
     print("%d roots in 64 media (%d checked by the oracle), sweeps FP64 %d / mixed %d, worst Newton step %.1e |k|"
           % (nroots, nsmall, sw64, sw32, worst))
     assert nroots >= 10            # 19 with these starts (4 per medium over Re k in [1, 40])
+
+def test_mixed_gated_map(G):
+    """The pivot-growth gate (MSGPU_MIXED_GATED) on the N=500 map: the flagged cells are the
+    ones whose complex64 factor's bound 16 eps32 exp(max log|U_ii| - min log|U_ii|) exceeds
+    tol; they are the complex128 values and the others the complex64 ones; every kept cell
+    is within tol of complex128; at tol = 1e-8 every cell is recomputed."""
+    med = G.Medium(3, 500, G.HARDSPHERE, (0.05,)).fill(3, radius=1.0)
+    ks = kgrid(24, 24, 5.0, 30.0, -2.0, 0.0)
+    f64, _ = G.tracelog_batch(med, ks, G.FP64)
+    f32, st32 = G.tracelog_stats(med, ks, G.FP32)
+    bound = 16 * 2.0 ** -24 * np.exp(st32[:, 0] - st32[:, 1])
+    for tol in (1e-8, 3e-6, 1e-4):
+        out, flags = G.tracelog_batch(med, ks, G.MIXED_GATED, tol)
+        want_flags = ~(bound <= tol)
+        assert np.array_equal(flags.astype(bool), want_flags), tol
+        assert np.array_equal(out[want_flags], f64[want_flags])
+        assert np.array_equal(out[~want_flags], f32[~want_flags])
+        err = np.hypot(out.real - f64.real, wrapped(500 * (out.imag - f64.imag)) / 500)
+        print("tol %.0e: %d of %d cells recomputed, max error of the kept cells %.2e"
+              % (tol, want_flags.sum(), ks.size, err[~want_flags].max() if (~want_flags).any() else 0.0))
+        assert (err <= tol).all(), (tol, err.max())
+        if tol == 1e-8:
+            assert want_flags.all()
+    # the serving loop gives the same tile
+    srv = G.Server()
+    got = srv.wait(srv.submit_logdet(med, ks[:40], G.MIXED_GATED, 1e-4))[0]["tracelog"]
+    srv.close()
+    assert np.array_equal(got, G.tracelog_batch(med, ks[:40], G.MIXED_GATED, 1e-4)[0])
