@@ -150,6 +150,57 @@ def dropin_msmain(n, nx, ny):
         return {"evals_per_s": None, "threads": threads, "sample": "failed: %s" % ex}
 
 
+def run_msmain_conf(exe, conf_text, threads, section):
+    """Runs an msmain build on a configuration; returns the realtime (s) msmain writes for
+    the given section of its output."""
+    tmp = tempfile.mkdtemp(prefix="msbench_")
+    try:
+        conf = os.path.join(tmp, "t.conf")
+        with open(conf, "w") as f:
+            f.write(conf_text)
+        env = dict(os.environ, MKL_THREADING_LAYER="GNU", MKL_NUM_THREADS="1", OMP_NUM_THREADS=str(threads))
+        subprocess.run([exe, "-p", str(threads), conf], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, timeout=600)
+        for dp, _, files in os.walk(os.path.join(tmp, "out")):
+            for fn in files:
+                if fn.endswith(".dat"):
+                    cur = None
+                    for line in open(os.path.join(dp, fn)):
+                        line = line.strip()
+                        if line.startswith("["):
+                            cur = line
+                        elif cur == "[%s]" % section and line.startswith("realtime="):
+                            return float(line.split("=")[1])
+        raise RuntimeError("no realtime for [%s] in the output of %s" % (section, exe))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def dropin_wavefunction(n, grids):
+    """msmain's [wavefunction] task (the pixel loop of run_wave_function_omp calling
+    eval_total_wave once per pixel from every thread): the reference msmain (MKL, host
+    threads) against msmain relinked to the library (concurrent calls coalesced on the
+    GPU); d=3, N unit-density ball, seed 1, hardsphere 0.1, plane wave k=10."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "msmain")
+    gpu = os.path.join(ROOT, "oracle", "_ref", "msmain_gpu")
+    if not (os.path.exists(ref) and os.path.exists(gpu)):
+        return None
+    threads = host_cores()
+    res = {"threads": threads, "workload": "d=3 N=%d unit-density ball seed 1, hardsphere 0.1, plane wave k=10" % n}
+    for g in grids:
+        conf = ("[medium]\ndimension=3\nnatom=%d\nmodel=hardsphere 0.1\nshape=ball\nratio=1\n\n"
+                "[wavefunction]\nnseed=1\niseed=1\nk=10.+0i\nsource=plane\nnx=%d\nny=%d\nxrange=-8:8\n"
+                "yrange=-8:8\ncolor1=temperature div 5\ncolor2=temperature div 5\n" % (n, g, g))
+        try:
+            tr = run_msmain_conf(ref, conf, threads, "wavefunction")
+            tg = run_msmain_conf(gpu, conf, threads, "wavefunction")
+            res["grid_%d" % g] = {"reference_s": tr, "dropin_s": tg, "dropin_over_reference": tr / tg,
+                                  "dropin_pixels_per_s": g * g / tg}
+        except Exception as ex:
+            res["grid_%d" % g] = {"failed": str(ex)}
+    return res
+
+
 def serving_stream(G, nreq, rate, nmin, nmax, seed=5):
     """Poisson stream of mixed requests through the serving loop (msgpu_server_*).
 
@@ -366,6 +417,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the serialized pass of the map (one stream), for the rocprof kernel-trace run")
     ap.add_argument("--no-dropin", action="store_true", help="skip timing msmain relinked against the library")
+    ap.add_argument("--dropin-wave-grids", default="256",
+                    help="grid sizes of the drop-in [wavefunction] timing (comma-separated)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run this many device copies of 1 GiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -534,6 +587,9 @@ def main():
     dropin = None if args.no_dropin else dropin_msmain(n, 48, 48)
     if dropin:
         progress("drop-in msmain done")
+    dropin_wave = None if args.no_dropin else dropin_wavefunction(n, [int(x) for x in args.dropin_wave_grids.split(",") if x])
+    if dropin_wave:
+        progress("drop-in wavefunction done")
     line = {
         "metric": "log-det evals/s at N=1000 (mixed vs FP64, same GPU); p50/p99 resonance latency",
         "value": value, "unit": "log-det evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -561,6 +617,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "dropin_msmain": dropin,
+        "dropin_wavefunction": dropin_wave,
     }
     print(json.dumps(line))
 

@@ -549,9 +549,12 @@ MSGPU_API msgpu_cplx eval_incident_wave(msgpu_iwave* iw, int d, double* pos) {
     return 0.0;
 }
 
+// Called once per pixel by every thread of run_wave_function_omp (ms_core.c:629-640):
+// concurrent calls on the same content run as one launch.
 MSGPU_API msgpu_cplx eval_total_wave(msgpu_iwave* iw, msgpu_medium* med, msgpu_cplx* a, double* pos) {
     msgpu_cplx psi;
-    die_on(wave_points(view_of(med), to_cx(iw->k), iw->source, as_cx(a), 1, pos, as_cx(&psi)), "eval_total_wave");
+    die_on(wave_point_coalesced(view_of(med), to_cx(iw->k), iw->source, as_cx(a), pos, as_cx(&psi)),
+           "eval_total_wave");
     return psi;
 }
 

@@ -113,6 +113,11 @@ int load_tuning(const char* path);
 // Device-resident copy of a medium's positions (released with release_positions).
 int upload_positions(const MediumView& mv, const double** dpos, size_t* token);
 void release_positions(const double* dpos, size_t token);
+// psi at one point, coalesced with the concurrent calls of other threads (same content of
+// positions, amplitudes, k and source) into one launch; the content's device copy is
+// reused from call to call.
+int wave_point_coalesced(const MediumView& mv, cx<double> k, int source, const cx<double>* a, const double* x,
+                         cx<double>* psi);
 // psi at arbitrary points (npt points, row-major [npt][d]).
 int wave_points(const MediumView& mv, cx<double> k, int source, const cx<double>* a, int npt, const double* pts,
                 cx<double>* psi);

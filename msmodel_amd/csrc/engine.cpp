@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -17,6 +18,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -1602,6 +1604,170 @@ int tracelog_host_coalesced(int n, cx<double>* a, int lda, int* ipiv, cx<double>
         unpack_block(a, lda, n, n, hs);
         *tracelog = req.tl;
         *info = req.info;
+        return 0;
+    });
+}
+
+// ------------------------------------------------------------------ coalesced psi points
+// Concurrent one-point evaluations of psi (the drop-in eval_total_wave, called once per
+// pixel by every thread of the caller's pixel loop) run as one launch.  The positions,
+// amplitudes, k and source of the calls are a "context": its device copy is made once
+// per content and reused while the calls keep passing the same content (each call
+// compares its inputs with the context's host copy, outside the lock).  A call queues its
+// point; a waiting caller, while fewer than kWvBatches batches run, takes every queued
+// point of the first one's context (at most kWvMax), runs them as one batch (one launch
+// that reads the points from pinned host memory and writes psi there) and marks them done.
+namespace {
+struct WaveCtx {
+    int d = 0, n = 0, source = 0;
+    cx<double> k{0.0, 0.0};
+    std::vector<double> pos;
+    std::vector<cx<double>> a;
+    DBuf<double> dpos;
+    DBuf<cx<double>> da;
+    bool same(const MediumView& mv, cx<double> k_, int source_, const cx<double>* a_) const {
+        return d == mv.d && n == mv.n && source == source_ && k.x == k_.x && k.y == k_.y &&
+               memcmp(pos.data(), mv.pos, sizeof(double) * (size_t)n * d) == 0 &&
+               memcmp(a.data(), a_, sizeof(cx<double>) * (size_t)n) == 0;
+    }
+};
+struct WaveReq {
+    std::shared_ptr<WaveCtx> ctx;
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    cx<double> psi{0.0, 0.0};
+    std::string err;
+    std::atomic<bool> done{false};      // set last, with release order, by the batch's runner
+};
+// The queue is guarded by g_wv_mu; the counters are read without it by the waiting callers,
+// which spin (yielding) on their own request's flag rather than sleeping on a condition
+// variable: a batch takes about as long as a thread wake-up, so sleeping callers would
+// spend most of the time being woken.
+std::mutex g_wv_mu;
+std::vector<WaveReq*> g_wv_queue;
+std::shared_ptr<WaveCtx> g_wv_ctx;
+std::atomic<int> g_wv_queued{0};       // g_wv_queue.size()
+std::atomic<int> g_wv_active{0};       // running batches
+std::atomic<int> g_wv_callers{0};      // threads inside wave_point_coalesced
+std::atomic<int> g_wv_hwm{0};          // the most callers seen at once since the last time none was inside
+std::atomic<int> g_wv_inflight{0};     // points in the running batches
+// Up to kWvBatches batches run at once, and a leader gathers until a kWvSplit-th of the
+// caller's threads (those not in a running batch, at most) have queued: the batches then
+// overlap the threads' return trips (best of 1-4 batches at once, split 1-4, measured with
+// 1, 4 and 16 calling threads at N = 1000 on an MI355X).
+constexpr int kWvBatches = 4;
+constexpr int kWvSplit = 4;
+constexpr int kWvMax = 4096;
+// A leader waits at most this long for the other callers to queue their points.
+constexpr auto kWvGather = std::chrono::microseconds(50);
+
+void run_wave_batch(const std::vector<WaveReq*>& B) {
+    Lease L;
+    const WaveCtx& c = *B[0]->ctx;
+    const int npt = (int)B.size(), d = c.d;
+    HostStage hp(((size_t)npt * d + 1) / 2), hq(npt);
+    double* hx = reinterpret_cast<double*>(hp.p);
+    for (int i = 0; i < npt; i++)
+        for (int q = 0; q < d; q++) hx[(size_t)i * d + q] = B[i]->x[q];
+    // the kernel reads the points from, and writes psi into, the pinned stages themselves
+    // (a few bytes per workgroup over the host link): no copy commands, so a batch costs one
+    // launch and one synchronisation
+    launch_wave_points(c.dpos.p, c.n, d, c.da.p, c.k, c.source, npt, hx, hq.p, L.st);
+    HIPCHK(hipGetLastError());
+    L.sync();
+    for (int i = 0; i < npt; i++) B[i]->psi = hq.p[i];
+}
+}  // namespace
+
+int wave_point_coalesced(const MediumView& mv, cx<double> k, int source, const cx<double>* a, const double* x,
+                         cx<double>* psi) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    return guarded([&]() -> int {
+        std::shared_ptr<WaveCtx> ctx;
+        {
+            std::lock_guard<std::mutex> g(g_wv_mu);
+            ctx = g_wv_ctx;
+        }
+        if (!ctx || !ctx->same(mv, k, source, a)) {        // a new content: its device copy
+            auto nc = std::make_shared<WaveCtx>();
+            nc->d = mv.d;
+            nc->n = mv.n;
+            nc->source = source;
+            nc->k = k;
+            nc->pos.assign(mv.pos, mv.pos + (size_t)mv.n * mv.d);
+            nc->a.assign(a, a + mv.n);
+            nc->dpos.alloc((size_t)mv.n * mv.d);
+            nc->da.alloc(mv.n);
+            Lease L;
+            HIPCHK(hipMemcpyAsync(nc->dpos.p, nc->pos.data(), sizeof(double) * nc->pos.size(), hipMemcpyHostToDevice,
+                                  L.st));
+            HIPCHK(hipMemcpyAsync(nc->da.p, nc->a.data(), sizeof(cx<double>) * mv.n, hipMemcpyHostToDevice, L.st));
+            L.sync();
+            std::lock_guard<std::mutex> g(g_wv_mu);
+            if (g_wv_ctx && g_wv_ctx->same(mv, k, source, a)) ctx = g_wv_ctx;   // another thread was first
+            else g_wv_ctx = ctx = nc;
+        }
+        WaveReq req;
+        req.ctx = ctx;
+        for (int q = 0; q < mv.d; q++) req.x[q] = x[q];
+        {
+            std::lock_guard<std::mutex> g(g_wv_mu);
+            const int c = ++g_wv_callers;
+            if (c > g_wv_hwm.load()) g_wv_hwm.store(c);
+            g_wv_queue.push_back(&req);
+            g_wv_queued.store((int)g_wv_queue.size());
+        }
+        while (!req.done.load(std::memory_order_acquire)) {
+            if (g_wv_active.load() >= kWvBatches || g_wv_queued.load() == 0) {
+                std::this_thread::yield();
+                continue;
+            }
+            // gather: the threads of the caller's loop that are not in a running batch are
+            // about to queue a point (the loops call again at once); wait for a share of
+            // them, briefly
+            const auto until = std::chrono::steady_clock::now() + kWvGather;
+            while (!req.done.load(std::memory_order_acquire) &&
+                   g_wv_queued.load() <
+                       std::min(g_wv_hwm.load() - g_wv_inflight.load(), (g_wv_hwm.load() + kWvSplit - 1) / kWvSplit) &&
+                   std::chrono::steady_clock::now() < until)
+                std::this_thread::yield();
+            std::vector<WaveReq*> batch;
+            {
+                std::lock_guard<std::mutex> g(g_wv_mu);
+                if (req.done.load(std::memory_order_acquire) || g_wv_active.load() >= kWvBatches ||
+                    g_wv_queue.empty())
+                    continue;
+                g_wv_active++;
+                const WaveCtx* bc = g_wv_queue.front()->ctx.get();
+                for (size_t i = 0; i < g_wv_queue.size();) {
+                    if (g_wv_queue[i]->ctx.get() == bc && (int)batch.size() < kWvMax) {
+                        batch.push_back(g_wv_queue[i]);
+                        g_wv_queue.erase(g_wv_queue.begin() + (long)i);
+                    } else {
+                        i++;
+                    }
+                }
+                g_wv_queued.store((int)g_wv_queue.size());
+                g_wv_inflight += (int)batch.size();
+            }
+            std::string err;
+            try {
+                run_wave_batch(batch);
+            } catch (const std::exception& ex) {
+                err = ex.what();
+            }
+            g_wv_inflight -= (int)batch.size();
+            g_wv_active--;
+            for (WaveReq* r : batch) {           // r belongs to its caller once done is set
+                r->err = err;
+                r->done.store(true, std::memory_order_release);
+            }
+        }
+        {
+            std::lock_guard<std::mutex> g(g_wv_mu);
+            if (--g_wv_callers == 0) g_wv_hwm.store(0);
+        }
+        if (!req.err.empty()) throw std::runtime_error(req.err);
+        *psi = req.psi;
         return 0;
     });
 }

@@ -373,6 +373,53 @@ __global__ __launch_bounds__(256) void wave_kernel(const double* __restrict__ po
     psi[l] = acc + inc;
 }
 
+// psi at a few points (the coalesced one-point calls): one workgroup per point, its
+// threads splitting the scatterers, then a block reduction, so that a small batch costs a
+// few Green evaluations per thread instead of N in one thread.
+__global__ __launch_bounds__(256) void wave_points_kernel(const double* __restrict__ pos, int n, int d,
+                                                          const cx<double>* __restrict__ a, cx<double> k,
+                                                          int source, const double* __restrict__ pts,
+                                                          cx<double>* __restrict__ psi) {
+    __shared__ double sre[256], sim[256];
+    const int p = blockIdx.x, tid = threadIdx.x;
+    double pt[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) pt[c] = c < d ? pts[(size_t)p * d + c] : 0.0;
+    cx<double> acc{0.0, 0.0};
+    for (int q = tid; q < n; q += 256) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const double e = (c < d ? pos[(size_t)q * d + c] : 0.0) - pt[c];
+            s += e * e;
+        }
+        acc = acc + a[q] * green<double>(d, k, sqrt(s));
+    }
+    sre[tid] = acc.x;
+    sim[tid] = acc.y;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) { sre[tid] += sre[tid + h]; sim[tid] += sim[tid + h]; }
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    cx<double> inc;
+    if (source == 0) {
+        inc = cexp(cx<double>{-k.y * pt[0], k.x * pt[0]});
+    } else {
+        double s = 0.0;
+        for (int c = 0; c < d; c++) s += pt[c] * pt[c];
+        inc = green<double>(d, k, sqrt(s));
+    }
+    psi[p] = cx<double>{sre[0], sim[0]} + inc;
+}
+
+void launch_wave_points(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int npt,
+                        const double* pts, cx<double>* psi, hipStream_t st) {
+    if (npt > 0)
+        hipLaunchKernelGGL(wave_points_kernel, dim3(npt), dim3(256), 0, st, pos, n, d, a, k, source, pts, psi);
+}
+
 void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int nx, int ny,
                  double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi, hipStream_t st) {
     long long tot = pts ? (long long)nx : (long long)nx * ny;

@@ -41,6 +41,9 @@ void launch_assemble(const double* pos, int n, int d, int np, const cx<double>* 
                      const cx<double>* izero, int kind, cx<T>* out, long long mstride, int nbatch, hipStream_t st);
 void launch_incident(const double* pos, int n, int d, int np, cx<double> k, int source, double ct, double st_,
                      cx<double>* phi, hipStream_t st);
+// psi at npt points (row-major [npt][d]), one workgroup per point (small batches)
+void launch_wave_points(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int npt,
+                        const double* pts, cx<double>* psi, hipStream_t st);
 // pts == nullptr: nx x ny grid; otherwise nx points given row-major [nx][d] (ny ignored)
 void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int nx, int ny,
                  double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi, hipStream_t st);
