@@ -125,6 +125,19 @@ template <class T> struct DBuf {
     }
 };
 
+// A pinned host buffer of count complex128 entries from the pinned pool.
+struct HostStage {
+    cx<double>* p = nullptr;
+    size_t cls = 0;
+    explicit HostStage(size_t count) { p = (cx<double>*)pinned_pool().get(sizeof(cx<double>) * count, cls); }
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    ~HostStage() {
+        drain_if_unwinding();
+        pinned_pool().put(p, cls);
+    }
+};
+
 // ------------------------------------------------------------------ streams
 class StreamPool {
     std::mutex mu_;
@@ -219,7 +232,7 @@ static int guarded(F&& f) {
 // ------------------------------------------------------------------ kernel timing
 // When enabled, every launch of the batched pipelines is bracketed by HIP events on
 // its own stream; categories: 0 panel, 1 swap+trsm, 2 gemm, 3 assembly, 4 tracelog,
-// 5 solve kernels, 6 other.  flops/bytes are the algorithmic counts of the launch.
+// 5 psi grid.  flops/bytes are the algorithmic counts of the launch.
 struct ProfRec {
     int cat;
     hipEvent_t a, b;
@@ -1320,11 +1333,49 @@ int eigenstates(const MediumView& mv, int nr, const cx<double>* kr, int maxit, d
 }
 
 // ------------------------------------------------------------------ psi
+// The grid is computed in chunks of rows (about 4 MiB of psi each) on one stream; a second
+// stream copies each finished chunk into one of kWaveSlots pinned buffers while later
+// chunks compute, and the host moves it from there into the caller's memory, so the
+// download and the host-side copy overlap the kernels instead of following them.
+namespace {
+constexpr int kWaveSlots = 4;
+constexpr size_t kWaveChunk = (size_t)1 << 18;          // points per chunk (4 MiB of psi, 1024 workgroups)
+// memcpy split over up to 8 threads for large blocks: a copy into fresh pageable memory
+// runs at the rate its page faults are served, and several threads take them in parallel.
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kPiece = (size_t)1 << 20;
+    const int nt = (int)std::min<size_t>(8, bytes / kPiece);
+    if (nt <= 1) { memcpy(dst, src, bytes); return; }
+    const size_t per = (bytes + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) {
+        const size_t o = per * t, len = std::min(per, bytes - o);
+        th.emplace_back([=] { memcpy((char*)dst + o, (const char*)src + o, len); });
+    }
+    memcpy(dst, src, per);
+    for (auto& x : th) x.join();
+}
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    explicit EventSet(int count) : ev(count, nullptr) {
+        for (auto& e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    EventSet(const EventSet&) = delete;
+    EventSet& operator=(const EventSet&) = delete;
+    ~EventSet() {
+        drain_if_unwinding();
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+}  // namespace
+
 int wave_grid(const MediumView& mv, cx<double> k, int source, const cx<double>* a, int nx, int ny, double xmin,
               double xmax, double ymin, double ymax, cx<double>* psi) {
     if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    if (nx <= 0 || ny <= 0) return 0;
     return guarded([&]() -> int {
-        Lease L;
+        Lease L, LC;
         const int n = mv.n;
         const size_t tot = (size_t)nx * ny;
         DBuf<double> dpos((size_t)n * mv.d);
@@ -1332,10 +1383,36 @@ int wave_grid(const MediumView& mv, cx<double> k, int source, const cx<double>* 
         HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpyAsync(da.p, a, sizeof(cx<double>) * n, hipMemcpyHostToDevice, L.st));
         const double dx = nx > 1 ? (xmax - xmin) / (nx - 1) : 0.0, dy = ny > 1 ? (ymax - ymin) / (ny - 1) : 0.0;
-        launch_wave(dpos.p, n, mv.d, da.p, k, source, nx, ny, xmin, dx, ymax, dy, nullptr, dpsi.p, L.st);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(psi, dpsi.p, sizeof(cx<double>) * tot, hipMemcpyDeviceToHost, L.st));
+        const int rows = (int)std::max<size_t>(1, std::min<size_t>((size_t)ny, kWaveChunk / (size_t)nx));
+        const int nch = (ny + rows - 1) / rows;
+        EventSet done(nch), copied(nch);
+        for (int c = 0; c < nch; c++) {
+            const int r0 = c * rows, rc = std::min(rows, ny - r0);
+            timed(5, 0.0, sizeof(cx<double>) * (double)rc * nx, L.st, [&] {
+                launch_wave(dpos.p, n, mv.d, da.p, k, source, nx, rc, r0, xmin, dx, ymax, dy, nullptr,
+                            dpsi.p + (size_t)r0 * nx, L.st);
+            });
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(done.ev[c], L.st));
+        }
+        const int ns = std::min(kWaveSlots, nch);
+        std::vector<std::unique_ptr<HostStage>> slot;
+        for (int i = 0; i < ns; i++) slot.emplace_back(new HostStage((size_t)rows * nx));
+        auto chunk_bytes = [&](int c) { return sizeof(cx<double>) * (size_t)std::min(rows, ny - c * rows) * nx; };
+        auto enqueue_copy = [&](int c) {
+            HIPCHK(hipStreamWaitEvent(LC.st, done.ev[c], 0));
+            HIPCHK(hipMemcpyAsync(slot[c % ns]->p, dpsi.p + (size_t)c * rows * nx, chunk_bytes(c),
+                                  hipMemcpyDeviceToHost, LC.st));
+            HIPCHK(hipEventRecord(copied.ev[c], LC.st));
+        };
+        for (int c = 0; c < ns; c++) enqueue_copy(c);
+        for (int c = 0; c < nch; c++) {
+            HIPCHK(hipEventSynchronize(copied.ev[c]));
+            parallel_copy(psi + (size_t)c * rows * nx, slot[c % ns]->p, chunk_bytes(c));
+            if (c + ns < nch) enqueue_copy(c + ns);
+        }
         L.sync();
+        LC.sync();
         return 0;
     });
 }
@@ -1351,7 +1428,7 @@ int wave_points(const MediumView& mv, cx<double> k, int source, const cx<double>
         HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpyAsync(dpts.p, pts, sizeof(double) * npt * mv.d, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpyAsync(da.p, a, sizeof(cx<double>) * n, hipMemcpyHostToDevice, L.st));
-        launch_wave(dpos.p, n, mv.d, da.p, k, source, npt, 1, 0.0, 0.0, 0.0, 0.0, dpts.p, dpsi.p, L.st);
+        launch_wave(dpos.p, n, mv.d, da.p, k, source, npt, 1, 0, 0.0, 0.0, 0.0, 0.0, dpts.p, dpsi.p, L.st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(psi, dpsi.p, sizeof(cx<double>) * npt, hipMemcpyDeviceToHost, L.st));
         L.sync();
@@ -1364,17 +1441,6 @@ int wave_points(const MediumView& mv, cx<double> k, int source, const cx<double>
 // calling thread packs or unpacks the block with memcpy, and the device copies the
 // packed block at the link's rate.  (A pitched copy straight from pageable memory is
 // staged by the runtime and serializes the threads that call concurrently.)
-struct HostStage {
-    cx<double>* p = nullptr;
-    size_t cls = 0;
-    explicit HostStage(size_t count) { p = (cx<double>*)pinned_pool().get(sizeof(cx<double>) * count, cls); }
-    HostStage(const HostStage&) = delete;
-    HostStage& operator=(const HostStage&) = delete;
-    ~HostStage() {
-        drain_if_unwinding();
-        pinned_pool().put(p, cls);
-    }
-};
 // stage <- host src (ld lds), rows x cols, packed (ld rows)
 static void pack_block(const HostStage& hs, const cx<double>* src, int lds, int rows, int cols) {
     const size_t rb = sizeof(cx<double>) * rows;

@@ -48,6 +48,24 @@ void launch_item_diag(MatItem* items, int cnt, hipStream_t st) {
     if (cnt > 0) hipLaunchKernelGGL(item_diag_kernel, dim3((cnt + 63) / 64), dim3(64), 0, st, items, cnt);
 }
 
+// ------------------------------------------------------------------ d = 3 Green function
+__device__ __forceinline__ double rsqrt_t(double x) { return rsqrt(x); }
+__device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
+
+// -G_3(k, r) = e^{ikr} / (4 pi r) from s = r^2, with 1/r from one reciprocal square root
+// and r = s * (1/r), in place of a square root and a division; REALK (Im k = 0) leaves out
+// the factor exp(-Im k r) = 1.  At s = 0 it returns what the general form does there.
+template <class T, bool REALK>
+__device__ __forceinline__ cx<T> neg_green3_s(cx<T> k, T s) {
+    const T ri = rsqrt_t(s);
+    const T r = s > T(0) ? s * ri : T(0);
+    T sn, cs;
+    sincos_t(k.x * r, &sn, &cs);
+    T w = ri * T(1.0 / kFourPi);
+    if (!REALK) w *= exp(-k.y * r);
+    return {cs * w, sn * w};
+}
+
 // ------------------------------------------------------------------ assembly
 // One workgroup per pair of 32x32 tiles (I <= J) of one matrix: it evaluates the
 // Green function once per unordered pair and writes both (I,J) and, through an
@@ -104,6 +122,14 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
             v = cx<T>{gi == gj ? T(1) : T(0), T(0)};
         } else if (gi == gj) {
             v = cx<T>{(T)dg.x, (T)dg.y};
+        } else if (d == 3 && kind == 0) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double e = sx[0][ii][c] - sx[1][jj][c];
+                s += e * e;
+            }
+            v = neg_green3_s<T, false>(k, (T)s);
         } else {
             double s = 0.0;
             for (int c = 0; c < d; c++) {
@@ -331,16 +357,19 @@ void launch_plane_items(const MatItem* items, int np, cx<double>* phi, int nbatc
 // grid point; scatterers and amplitudes are staged through LDS in chunks.
 constexpr int WCH = 256;
 
+// D3: d = 3 through neg_green3_s (REALK: Im k = 0); otherwise the general Green function.
+template <bool D3, bool REALK>
 __global__ __launch_bounds__(256) void wave_kernel(const double* __restrict__ pos, int n, int d,
                                                    const cx<double>* __restrict__ a, cx<double> k, int source,
-                                                   int nx, int ny, double xmin, double dx, double ymax, double dy,
-                                                   const double* __restrict__ pts, cx<double>* __restrict__ psi) {
+                                                   int nx, int ny, int row0, double xmin, double dx, double ymax,
+                                                   double dy, const double* __restrict__ pts,
+                                                   cx<double>* __restrict__ psi) {
     __shared__ double sp[WCH][4];
     __shared__ cx<double> sa[WCH];
     const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = pts ? (long long)nx : (long long)nx * ny;
     const bool live = l < tot;
-    const int ix = live ? (int)(l % nx) : 0, iy = live ? (int)(l / nx) : 0;
+    const int ix = live ? (int)(l % nx) : 0, iy = live ? row0 + (int)(l / nx) : 0;
     double pt[4] = {xmin + ix * dx, (d > 1) ? ymax - iy * dy : 0.0, 0.0, 0.0};
     if (pts && live)
         for (int c = 0; c < d; c++) pt[c] = pts[(size_t)l * d + c];
@@ -354,14 +383,22 @@ __global__ __launch_bounds__(256) void wave_kernel(const double* __restrict__ po
         }
         __syncthreads();
         if (live) {
-            for (int q = 0; q < cnt; q++) {
-                double s = 0.0;
-                for (int c = 0; c < d; c++) { double e = sp[q][c] - pt[c]; s += e * e; }
-                acc = acc + sa[q] * green<double>(d, k, sqrt(s));
+            if constexpr (D3) {
+                for (int q = 0; q < cnt; q++) {
+                    const double e0 = sp[q][0] - pt[0], e1 = sp[q][1] - pt[1], e2 = sp[q][2] - pt[2];
+                    acc = acc + sa[q] * neg_green3_s<double, REALK>(k, e0 * e0 + e1 * e1 + e2 * e2);
+                }
+            } else {
+                for (int q = 0; q < cnt; q++) {
+                    double s = 0.0;
+                    for (int c = 0; c < d; c++) { double e = sp[q][c] - pt[c]; s += e * e; }
+                    acc = acc + sa[q] * green<double>(d, k, sqrt(s));
+                }
             }
         }
     }
     if (!live) return;
+    if constexpr (D3) acc = cx<double>{-acc.x, -acc.y};      // G_3 = -(e^{ikr} / (4 pi r))
     cx<double> inc;
     if (source == 0) {
         inc = cexp(cx<double>{-k.y * pt[0], k.x * pt[0]});
@@ -393,7 +430,7 @@ __global__ __launch_bounds__(256) void wave_points_kernel(const double* __restri
             const double e = (c < d ? pos[(size_t)q * d + c] : 0.0) - pt[c];
             s += e * e;
         }
-        acc = acc + a[q] * green<double>(d, k, sqrt(s));
+        acc = acc + a[q] * (d == 3 ? -neg_green3_s<double, false>(k, s) : green<double>(d, k, sqrt(s)));
     }
     sre[tid] = acc.x;
     sim[tid] = acc.y;
@@ -421,10 +458,12 @@ void launch_wave_points(const double* pos, int n, int d, const cx<double>* a, cx
 }
 
 void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int nx, int ny,
-                 double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi, hipStream_t st) {
+                 int row0, double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi,
+                 hipStream_t st) {
     long long tot = pts ? (long long)nx : (long long)nx * ny;
-    hipLaunchKernelGGL(wave_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, pos, n, d, a, k, source, nx,
-                       ny, xmin, dx, ymax, dy, pts, psi);
+    auto kern = d != 3 ? wave_kernel<false, false> : k.y == 0.0 ? wave_kernel<true, true> : wave_kernel<true, false>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, pos, n, d, a, k, source, nx, ny,
+                       row0, xmin, dx, ymax, dy, pts, psi);
 }
 
 }  // namespace msg

@@ -44,9 +44,11 @@ void launch_incident(const double* pos, int n, int d, int np, cx<double> k, int 
 // psi at npt points (row-major [npt][d]), one workgroup per point (small batches)
 void launch_wave_points(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int npt,
                         const double* pts, cx<double>* psi, hipStream_t st);
-// pts == nullptr: nx x ny grid; otherwise nx points given row-major [nx][d] (ny ignored)
+// pts == nullptr: rows row0 .. row0+ny-1 of a grid nx wide (psi holds those rows);
+// otherwise nx points given row-major [nx][d] (ny and row0 ignored)
 void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int nx, int ny,
-                 double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi, hipStream_t st);
+                 int row0, double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi,
+                 hipStream_t st);
 
 // kern_lu.hip
 // rm: the matrices are stored row-major (see at<> in kern_lu.hip); variant: panel

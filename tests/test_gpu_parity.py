@@ -233,6 +233,26 @@ def test_solve_and_wavefunction_vs_reference(G, ref, tag, model, p, k, grid, sou
     assert abs(one - want1) < 1e-10 * abs(want1)
 
 
+@pytest.mark.parametrize("d,k,grid", [
+    (3, 10.0, (300, 3000, -3, 3, -4, 4)),         # real k: four chunks of rows, the last one ragged
+    (3, 7 - 0.3j, (257, 300, -2, 2, -3, 1)),      # complex k
+    (2, 6 - 0.2j, (270001, 2, -5, 5, -1, 1)),     # rows wider than a chunk: one row per chunk
+    (1, 3 + 0.0j, (5, 3, -2, 2, 0, 0))])          # a grid smaller than one chunk
+def test_wavefunction_grid_chunks_vs_oracle(G, d, k, grid):
+    """psi grids computed in chunks of rows (downloaded while later chunks compute)
+    against the oracle's eval_total_wave over get_pos_wfun's grid (incident_wave_util.c:87-96,
+    wave_function_util.c:76-81), d = 1..3, real and complex k, ragged chunks."""
+    pos = O.fill_medium(d, 30, 4)
+    m = medium(G, pos, HS, (0.1,))
+    a, _ = G.solve(m, k, O.PLANE, G.FP64)
+    psi = G.wavefunction_grid(m, k, a, *grid)
+    want = O.wavefunction_grid(pos, a, k, *grid)
+    assert psi.shape == want.shape
+    err = np.abs(psi - want).max() / np.abs(want).max()
+    print("grid %s: max relative error %.2e" % (grid[:2], err))
+    assert err < 1e-10
+
+
 def test_mixed_solve_refines_to_fp64(G, ref):
     pos = ref["sol_cfg2_pos"]
     m = medium(G, pos, HS, (0.1,))
