@@ -264,7 +264,28 @@ def serving_stream(G, nreq, rate, nmin, nmax, seed=5):
             "by_kind_ms": {"logdet_tile64": pct(lat[0]), "solve_psi256": pct(lat[1]), "resonance32": pct(lat[2])},
             "requests_with_N_ge_4000_by_kind": {"logdet_tile64": nbig[0], "solve_psi256": nbig[1], "resonance32": nbig[2]},
             "steps": st["steps"], "batched_matrices": st["batched_matrices"],
-            "max_requests_per_step": st["max_requests_per_step"]}
+            "max_requests_per_step": st["max_requests_per_step"], "resident_media_at_end": st["resident_media"],
+            "kept_factor_hits": st["factor_hits"], "kept_factor_misses": st["factor_misses"]}
+
+
+def kept_factor_solves(G, n=2000):
+    """Complex128 solves + 256^2 psi through the server over one medium (d=3, N unit-density
+    ball, seed 3, alpha=0.1) at one k = 8 - 0.05i: the first factors M(k) and keeps the
+    factor on the device; the next ones (other sources and grids) solve with it."""
+    med = G.Medium(3, n, G.HARDSPHERE, (0.1,)).fill(3)
+    r = float(np.abs(med.pos).max()) + 0.5
+    grids = [(256, 256, -r, r, -r, r), (256, 256, -r / 2, r / 2, -r / 2, r / 2)]
+    srv = G.Server(hbm_gb=64.0)
+    lat = []
+    for j, (src, grid) in enumerate([(G.PLANE, grids[0]), (G.SPHERICAL, grids[0]), (G.PLANE, grids[1]),
+                                     (G.SPHERICAL, grids[1])]):
+        lat.append(srv.wait(srv.submit_solve(med, 8.0 - 0.05j, src, G.FP64, 0.0, grid))[1]["total_ms"])
+    st = srv.stats()
+    srv.close()
+    return {"workload": "d=3 N=%d unit-density ball seed 3, alpha 0.1, k = 8 - 0.05i, complex128 solve + 256^2 "
+                        "psi, sources plane/spherical, two grids" % n,
+            "first_ms": lat[0], "with_kept_factor_ms": lat[1:], "factor_hits": st["factor_hits"],
+            "factor_misses": st["factor_misses"], "kept_factor_bytes": st["kept_factor_bytes"]}
 
 
 def uneven_media(G, nmed, total, nmin, nmax):
@@ -539,8 +560,11 @@ def main():
         uneven = uneven_search(G, args.uneven, args.uneven_starts, 200, 2000)
         progress("uneven search: %d starts in %.1f s" % (uneven["starts"], uneven["wall_s"]))
         progress("uneven resonance search done")
+    kept = None
     if not args.no_wave:
         wave = wavefunction_case(G)
+        kept = kept_factor_solves(G)
+        progress("kept-factor solves done")
         progress("solve + psi grid done")
     if not args.no_small:
         small = small_map(G, 3, args.tune)
@@ -619,6 +643,7 @@ def main():
         "serving": serving,
         "uneven_resonance_search": uneven,
         "solve_wavefunction_n1000": wave,
+        "server_kept_factor_solves": kept,
         "map_n200": small,
         "kernel_time_fraction": breakdown,
         "autotune": tuned,

@@ -313,7 +313,8 @@ long long msgpu_server_submit(msgpu_server* s, const msgpu_request* r);
 /* Blocks until the request is done; returns its status.  Each id is waited for once. */
 int msgpu_server_wait(msgpu_server* s, long long id, msgpu_result* res);
 /* stats[0..nstats): steps, batched matrices, most requests in one step, peak admitted
- * bytes, budget bytes. */
+ * bytes (kept factors included), budget bytes, complex128 solves that used a kept factor,
+ * complex128 solves that factored M(k), bytes of kept factors now, resident media now. */
 int msgpu_server_stats(msgpu_server* s, double* stats, int nstats);
 /* Serves every submitted request, then stops the loops and frees the server. */
 void msgpu_server_destroy(msgpu_server* s);

@@ -557,10 +557,11 @@ class Server:
         return outs, {"count": res.count, "queue_ms": res.queue_ms, "total_ms": res.total_ms}
 
     def stats(self):
-        v = np.zeros(5)
-        lib.msgpu_server_stats(self.h, _cp(v), 5)
+        v = np.zeros(9)
+        lib.msgpu_server_stats(self.h, _cp(v), 9)
         return {"steps": int(v[0]), "batched_matrices": int(v[1]), "max_requests_per_step": int(v[2]),
-                "peak_admitted_bytes": v[3], "budget_bytes": v[4]}
+                "peak_admitted_bytes": v[3], "budget_bytes": v[4], "factor_hits": int(v[5]),
+                "factor_misses": int(v[6]), "kept_factor_bytes": v[7], "resident_media": int(v[8])}
 
     def close(self):
         if self.h:

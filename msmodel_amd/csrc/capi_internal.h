@@ -5,6 +5,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -83,6 +84,16 @@ int dk_log_det_multi(const std::vector<MediumView>& meds, int nk, const int* mid
 int dk_log_det_points(const MediumView& mv, int nk, const cx<double>* ks, cx<double>* out, int precision);
 // Amplitudes a = M(k)^-1 phi; refine: complex64 LU + FP64 refinement to tol.  Returns #steps or <0.
 int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>* a, int mixed, double tol);
+// A complex128 factor of one M(k) kept on the device: the row-major LU of the symmetric M
+// (left interchanges applied) and the row map P of M^-1 v = P V.
+struct ResidentFactor;
+size_t resident_factor_bytes(const ResidentFactor& f);
+// The complex128 solve of solve_amplitudes (same kernels and launch order, so the same
+// amplitudes bit for bit).  With reuse, that factor is used: M(k) is neither assembled nor
+// factored.  With keep, the factor used is handed back for later calls.  mv.dpos, when set,
+// is a device copy of the positions.
+int solve_resident(const MediumView& mv, cx<double> k, int source, cx<double>* a,
+                   const std::shared_ptr<ResidentFactor>& reuse, std::shared_ptr<ResidentFactor>* keep);
 int eigenstates(const MediumView& mv, int nr, const cx<double>* kr, int maxit, double toler, cx<double>* vecs,
                 cx<double>* mu, int* conv, const cx<double>* v_init = nullptr);
 int trace_inv_multi(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,

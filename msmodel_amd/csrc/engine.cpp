@@ -1198,6 +1198,62 @@ int solve_amplitudes(const MediumView& mv, cx<double> k, int source, cx<double>*
     });
 }
 
+// ------------------------------------------------------------------ resident factors
+struct ResidentFactor {
+    int np = 0, info = 0;
+    DBuf<cx<double>> lu;
+    DBuf<int> perm;
+};
+size_t resident_factor_bytes(const ResidentFactor& f) {
+    return sizeof(cx<double>) * (size_t)f.np * f.np + sizeof(int) * (size_t)f.np;
+}
+
+int solve_resident(const MediumView& mv, cx<double> k, int source, cx<double>* a,
+                   const std::shared_ptr<ResidentFactor>& reuse, std::shared_ptr<ResidentFactor>* keep) {
+    if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
+    if (reuse && reuse->np != mv.np) { set_error("msgpu: resident factor of another size"); return -2; }
+    return guarded([&]() -> int {
+        Lease L;
+        const int np = mv.np, n = mv.n;
+        const long long ms = (long long)np * np;
+        DBuf<double> own;
+        const double* dpos = mv.dpos;
+        if (!dpos) {
+            own.alloc((size_t)n * mv.d);
+            HIPCHK(hipMemcpyAsync(own.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+            dpos = own.p;
+        }
+        DBuf<cx<double>> phi(np), x(np);
+        std::shared_ptr<ResidentFactor> f = reuse;
+        if (!f) {
+            f = std::make_shared<ResidentFactor>();
+            f->np = np;
+            f->lu.alloc(ms);
+            f->perm.alloc(np);
+            DBuf<cx<double>> dk(1), ddiag(1);
+            DBuf<int> ipiv(np), info(1), pmap(factor_ws_ints(np, NB, 1));
+            HIPCHK(hipMemcpyAsync(dk.p, &k, sizeof(k), hipMemcpyHostToDevice, L.st));
+            launch_diag(dk.p, 1, mv.m, mv.d, 0, ddiag.p, nullptr, L.st);
+            launch_incident(dpos, n, mv.d, np, k, source, 1.0, 0.0, phi.p, L.st);
+            launch_assemble<double>(dpos, n, mv.d, np, dk.p, ddiag.p, nullptr, 0, f->lu.p, ms, 1, L.st);
+            factor<double>(f->lu.p, np, 1, ipiv.p, info.p, pmap.p, true, true, L.st);
+            launch_build_perm(pmap.p, np, NB, f->perm.p, 1, L.st);
+            HIPCHK(hipMemcpyAsync(&f->info, info.p, sizeof(int), hipMemcpyDeviceToHost, L.st));
+        } else {
+            launch_incident(dpos, n, mv.d, np, k, source, 1.0, 0.0, phi.p, L.st);
+        }
+        lu_solve_rm<double>(f->lu.p, phi.p, np, 1, 1, np, L.st);
+        launch_scatter_rows<double>(x.p, phi.p, f->perm.p, np, 1, np, np, 1, L.st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(a, x.p, sizeof(cx<double>) * n, hipMemcpyDeviceToHost, L.st));
+        L.sync();
+        if (f->info > 0)
+            printf("[LAPACK] Element U(%d,%d) is exactly zero, the solution could not be computed.\n", f->info, f->info);
+        if (keep) *keep = f;
+        return 0;
+    });
+}
+
 // ------------------------------------------------------------------ amplitudes of many items
 // a = M(k)^-1 phi for items (medium, k) with phi the plane wave along 1x: grouped and
 // chunked like the trace-log batches; each chunk assembled, factored row-major with left
@@ -1378,9 +1434,14 @@ int wave_grid(const MediumView& mv, cx<double> k, int source, const cx<double>* 
         Lease L, LC;
         const int n = mv.n;
         const size_t tot = (size_t)nx * ny;
-        DBuf<double> dpos((size_t)n * mv.d);
+        DBuf<double> own;
+        const double* dpos = mv.dpos;          // a resident copy of the positions, if the caller keeps one
+        if (!dpos) {
+            own.alloc((size_t)n * mv.d);
+            HIPCHK(hipMemcpyAsync(own.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
+            dpos = own.p;
+        }
         DBuf<cx<double>> da(n), dpsi(tot);
-        HIPCHK(hipMemcpyAsync(dpos.p, mv.pos, sizeof(double) * n * mv.d, hipMemcpyHostToDevice, L.st));
         HIPCHK(hipMemcpyAsync(da.p, a, sizeof(cx<double>) * n, hipMemcpyHostToDevice, L.st));
         const double dx = nx > 1 ? (xmax - xmin) / (nx - 1) : 0.0, dy = ny > 1 ? (ymax - ymin) / (ny - 1) : 0.0;
         const int rows = (int)std::max<size_t>(1, std::min<size_t>((size_t)ny, kWaveChunk / (size_t)nx));
@@ -1389,7 +1450,7 @@ int wave_grid(const MediumView& mv, cx<double> k, int source, const cx<double>* 
         for (int c = 0; c < nch; c++) {
             const int r0 = c * rows, rc = std::min(rows, ny - r0);
             timed(5, 0.0, sizeof(cx<double>) * (double)rc * nx, L.st, [&] {
-                launch_wave(dpos.p, n, mv.d, da.p, k, source, nx, rc, r0, xmin, dx, ymax, dy, nullptr,
+                launch_wave(dpos, n, mv.d, da.p, k, source, nx, rc, r0, xmin, dx, ymax, dy, nullptr,
                             dpsi.p + (size_t)r0 * nx, L.st);
             });
             HIPCHK(hipGetLastError());

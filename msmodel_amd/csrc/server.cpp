@@ -8,8 +8,15 @@
 //   runs (alone).  A later request that fits may pass one that does not, but only
 //   kMaxBypass times: then nothing behind the passed-over request is admitted until it
 //   fits (at the latest once the device has drained), so a large request is not starved
-//   by a steady stream of small ones.  An admitted request's positions are copied to
-//   the device once and stay resident until it finishes.
+//   by a steady stream of small ones.
+// * Residency.  Positions live on the device once per medium content: requests over the
+//   same medium share that copy, which stays resident after they finish (the kResidentMedia
+//   most recently used media).  A complex128 solve keeps its factored M(k) on the device,
+//   keyed by medium content, scattering model and k; a later solve at the same point (a
+//   different source, another psi grid) solves with it instead of assembling and factoring
+//   M(k) again, with the same kernels, so its amplitudes are the same bit for bit.  Kept
+//   factors count against the budget (at most half of it) and are dropped, least recently
+//   used first, when a request needs the room.
 // * Continuous batching.  The batch loop repeatedly forms one step from every
 //   admitted log-det and resonance request: each contributes the matrices of its
 //   current stage (all tile cells; the FP64 re-evaluation cells of a mixed tile; the
@@ -47,13 +54,50 @@ inline double ms_between(clk::time_point a, clk::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// The device copy of one medium's positions, shared by every request over the same content.
+struct ResidentMedium {
+    uint64_t id = 0;                    // never reused, keys the factors computed over it
+    uint64_t hash = 0;
+    int d = 0, n = 0;
+    std::vector<double> pos;
+    const double* dpos = nullptr;
+    size_t tok = 0;
+    int refs = 0;                       // admitted requests and kept factors using it
+    uint64_t last = 0;
+};
+
+// A kept complex128 factor of M(k) over a resident medium.
+struct KeptFactor {
+    ResidentMedium* med = nullptr;
+    ScModel m{};
+    cd k;
+    std::shared_ptr<ResidentFactor> f;
+    size_t bytes = 0;
+    uint64_t last = 0;
+};
+
+// FNV-1a over 64-bit words: a quick first test; equal hashes are confirmed byte for byte.
+uint64_t content_hash(int d, int n, const double* pos) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t w) { h ^= w; h *= 1099511628211ull; };
+    mix((uint64_t)d);
+    mix((uint64_t)n);
+    for (size_t i = 0; i < (size_t)n * d; i++) {
+        uint64_t w;
+        memcpy(&w, pos + i, sizeof w);
+        mix(w);
+    }
+    return h;
+}
+
+constexpr int kResidentMedia = 64;
+
 struct Req {
     long long id = 0;
     msgpu_request r{};                  // the caller's request; output pointers stay the caller's
     MediumView mv;
     std::vector<double> hpos;           // host copy of the positions
-    const double* dpos = nullptr;       // resident device copy while admitted
-    size_t dtok = 0;
+    ResidentMedium* rmed = nullptr;     // its resident device copy while admitted
     std::vector<cd> ks;                 // LOGDET: copy of the k points
     std::vector<double> stats;          // LOGDET, MSGPU_MIXED_GATED: the complex64 pivot statistics
     int stage = 0;                      // LOGDET: 0 first pass, 1 FP64 re-evaluation
@@ -81,11 +125,23 @@ struct Server {
     long long next_id = 0, steps = 0, items = 0, max_step_reqs = 0;
     bool stop = false;
     std::thread batch_thread, solve_thread;
+    std::vector<std::unique_ptr<ResidentMedium>> media;
+    std::vector<KeptFactor> kept;
+    size_t kept_bytes = 0;
+    uint64_t tick = 0, next_medium = 0;
+    long long factor_hits = 0, factor_misses = 0;
 
     void admit_locked();
     void batch_loop();
     void solve_loop();
     void finish_locked(Req* q);
+    ResidentMedium* acquire_medium_locked(const MediumView& mv);
+    void release_medium_locked(ResidentMedium* rm);
+    void drop_kept_locked(size_t i);
+    bool make_room_locked(size_t need);
+    std::shared_ptr<ResidentFactor> find_kept_locked(const Req* q);
+    void keep_factor_locked(const Req* q, std::shared_ptr<ResidentFactor> f);
+    ~Server();
 };
 
 namespace {
@@ -102,13 +158,109 @@ size_t footprint_of(const msgpu_request& r, int n) {
 
 }  // namespace
 
+ResidentMedium* Server::acquire_medium_locked(const MediumView& mv) {
+    const uint64_t h = content_hash(mv.d, mv.n, mv.pos);
+    const size_t cnt = (size_t)mv.n * mv.d;
+    for (auto& rm : media)
+        if (rm->hash == h && rm->d == mv.d && rm->n == mv.n && memcmp(rm->pos.data(), mv.pos, sizeof(double) * cnt) == 0) {
+            rm->refs++;
+            rm->last = ++tick;
+            return rm.get();
+        }
+    // drop unused media beyond the kept number, least recently used first
+    while ((int)media.size() >= kResidentMedia) {
+        int victim = -1;
+        for (int i = 0; i < (int)media.size(); i++)
+            if (media[i]->refs == 0 && (victim < 0 || media[i]->last < media[victim]->last)) victim = i;
+        if (victim < 0) break;
+        release_positions(media[victim]->dpos, media[victim]->tok);
+        media.erase(media.begin() + victim);
+    }
+    std::unique_ptr<ResidentMedium> rm(new ResidentMedium());
+    rm->id = ++next_medium;
+    rm->hash = h;
+    rm->d = mv.d;
+    rm->n = mv.n;
+    rm->pos.assign(mv.pos, mv.pos + cnt);
+    MediumView v = mv;
+    v.pos = rm->pos.data();
+    if (upload_positions(v, &rm->dpos, &rm->tok) < 0) return nullptr;
+    rm->refs = 1;
+    rm->last = ++tick;
+    media.push_back(std::move(rm));
+    return media.back().get();
+}
+
+void Server::release_medium_locked(ResidentMedium* rm) {
+    if (!rm) return;
+    rm->refs--;
+    rm->last = ++tick;
+}
+
+void Server::drop_kept_locked(size_t i) {
+    kept_bytes -= kept[i].bytes;
+    in_use -= kept[i].bytes;
+    release_medium_locked(kept[i].med);
+    kept.erase(kept.begin() + (long)i);         // the factor's memory goes when its last user lets go
+}
+
+// Drops kept factors, least recently used first, until need more bytes fit the budget.
+bool Server::make_room_locked(size_t need) {
+    while (in_use + need > budget && !kept.empty()) {
+        size_t v = 0;
+        for (size_t i = 1; i < kept.size(); i++)
+            if (kept[i].last < kept[v].last) v = i;
+        drop_kept_locked(v);
+    }
+    return in_use + need <= budget;
+}
+
+std::shared_ptr<ResidentFactor> Server::find_kept_locked(const Req* q) {
+    const cd k = q->r.k;
+    for (KeptFactor& e : kept)
+        if (e.med == q->rmed && e.k == k && e.m.type == q->mv.m.type && e.m.p0 == q->mv.m.p0 &&
+            e.m.p1 == q->mv.m.p1) {
+            e.last = ++tick;
+            return e.f;
+        }
+    return nullptr;
+}
+
+void Server::keep_factor_locked(const Req* q, std::shared_ptr<ResidentFactor> f) {
+    const size_t bytes = resident_factor_bytes(*f);
+    if (bytes > budget / 2) return;
+    while (kept_bytes + bytes > budget / 2 && !kept.empty()) {
+        size_t v = 0;
+        for (size_t i = 1; i < kept.size(); i++)
+            if (kept[i].last < kept[v].last) v = i;
+        drop_kept_locked(v);
+    }
+    if (!make_room_locked(bytes)) return;
+    KeptFactor e;
+    e.med = q->rmed;
+    e.med->refs++;
+    e.m = q->mv.m;
+    e.k = q->r.k;
+    e.f = std::move(f);
+    e.bytes = bytes;
+    e.last = ++tick;
+    kept.push_back(std::move(e));
+    kept_bytes += bytes;
+    in_use += bytes;
+    peak = std::max(peak, in_use);
+}
+
+Server::~Server() {
+    kept.clear();
+    for (auto& rm : media) release_positions(rm->dpos, rm->tok);
+}
+
 void Server::finish_locked(Req* q) {
     q->done = true;
     q->t_done = clk::now();
     in_use -= q->footprint;
     n_admitted--;
-    release_positions(q->dpos, q->dtok);
-    q->dpos = nullptr;
+    release_medium_locked(q->rmed);
     q->hpos.clear();
     q->hpos.shrink_to_fit();
     cv.notify_all();
@@ -122,14 +274,17 @@ void Server::admit_locked() {
     std::vector<Req*> passed;           // waiting requests ahead of the current one
     for (auto it = pending.begin(); it != pending.end();) {
         Req* q = *it;
-        const bool fits = in_use + q->footprint <= budget || n_admitted == 0;
+        bool fits = in_use + q->footprint <= budget;
+        if (!fits && n_admitted < max_active) fits = make_room_locked(q->footprint);
+        if (!fits && n_admitted == 0) fits = true;     // alone, a request larger than the budget still runs
         if (!fits || n_admitted >= max_active) {
             if (q->bypassed >= kMaxBypass) break;      // it has waited long enough: hold the rest
             passed.push_back(q);
             ++it;
             continue;
         }
-        if (upload_positions(q->mv, &q->dpos, &q->dtok) < 0) {
+        q->rmed = acquire_medium_locked(q->mv);
+        if (!q->rmed) {
             q->status = -2;
             q->err = last_error();
             q->done = true;
@@ -139,7 +294,7 @@ void Server::admit_locked() {
             continue;
         }
         for (Req* w : passed) w->bypassed++;
-        q->mv.dpos = q->dpos;
+        q->mv.dpos = q->rmed->dpos;
         q->t_admit = clk::now();
         in_use += q->footprint;
         peak = std::max(peak, in_use);
@@ -162,8 +317,25 @@ void Server::solve_loop() {
             solveq.pop_front();
         }
         const msgpu_request& r = q->r;
-        int rc = solve_amplitudes(q->mv, cx<double>{r.k.real(), r.k.imag()}, r.source, as_cx(r.a),
-                                  r.mode == MSGPU_FP64 ? 0 : 1, r.tol);
+        int rc;
+        if (r.mode == MSGPU_FP64) {            // through the kept factors
+            std::shared_ptr<ResidentFactor> f, made;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                f = find_kept_locked(q);
+                (f ? factor_hits : factor_misses)++;
+            }
+            rc = solve_resident(q->mv, cx<double>{r.k.real(), r.k.imag()}, r.source, as_cx(r.a), f,
+                                f ? nullptr : &made);
+            if (rc >= 0 && made) {
+                std::lock_guard<std::mutex> g(mu);
+                in_use -= q->footprint;        // the request's matrix becomes the kept factor
+                q->footprint = 0;
+                keep_factor_locked(q, std::move(made));
+            }
+        } else {
+            rc = solve_amplitudes(q->mv, cx<double>{r.k.real(), r.k.imag()}, r.source, as_cx(r.a), 1, r.tol);
+        }
         std::string err = rc < 0 ? last_error() : "";
         if (rc >= 0 && r.psi && r.nx > 0 && r.ny > 0) {
             const int rc2 = wave_grid(q->mv, cx<double>{r.k.real(), r.k.imag()}, r.source, as_cx(r.a), r.nx, r.ny,
@@ -388,9 +560,10 @@ MSGPU_API int msgpu_server_wait(msgpu_server* sv, long long id, msgpu_result* re
 MSGPU_API int msgpu_server_stats(msgpu_server* sv, double* stats, int nstats) {
     Server* s = reinterpret_cast<Server*>(sv);
     std::lock_guard<std::mutex> g(s->mu);
-    const double v[5] = {(double)s->steps, (double)s->items, (double)s->max_step_reqs, (double)s->peak,
-                         (double)s->budget};
-    for (int i = 0; i < nstats && i < 5; i++) stats[i] = v[i];
+    const double v[9] = {(double)s->steps, (double)s->items, (double)s->max_step_reqs, (double)s->peak,
+                         (double)s->budget, (double)s->factor_hits, (double)s->factor_misses,
+                         (double)s->kept_bytes, (double)s->media.size()};
+    for (int i = 0; i < nstats && i < 9; i++) stats[i] = v[i];
     return 0;
 }
 

@@ -223,3 +223,55 @@ def test_large_request_not_starved(G):
     print("small requests submitted after the large one and admitted before it:", passed)
     assert passed <= 6, passed                           # at most 4 bypasses (+ clock slack)
     assert np.isfinite(out_big["tracelog"]).all()
+
+
+def test_kept_factors_and_resident_media(G):
+    """Complex128 solves keep their factored M(k) on the device: a later solve over the same
+    medium, model and k (another source, another grid) uses it instead of factoring again,
+    with amplitudes and psi identical to the direct calls; media are shared by content."""
+    med = G.Medium(3, 300, G.HARDSPHERE, (0.1,)).fill(21)
+    twin = G.Medium(3, 300, G.HARDSPHERE, (0.1,)).fill(21)      # same content, another object
+    grid = (16, 12, -3.0, 3.0, -2.0, 2.0)
+    cases = [(med, 5.0 - 0.1j, G.PLANE), (twin, 5.0 - 0.1j, G.SPHERICAL), (med, 6.0 - 0.2j, G.PLANE),
+             (med, 5.0 - 0.1j, G.PLANE)]
+    srv = G.Server(hbm_gb=4.0)
+    outs = []
+    for m, k, src in cases:              # one after another: each finds the factors kept before it
+        rid = srv.submit_solve(m, k, src, G.FP64, 0.0, grid)
+        outs.append(srv.wait(rid)[0])
+    st = srv.stats()
+    srv.close()
+    for (m, k, src), out in zip(cases, outs):
+        aw, _ = G.solve(m, k, src, G.FP64)
+        assert np.array_equal(out["a"], aw)
+        assert np.array_equal(out["psi"], G.wavefunction_grid(m, k, aw, *grid, source=src).ravel())
+    npad = (300 + 63) // 64 * 64
+    print(st)
+    assert (st["factor_hits"], st["factor_misses"]) == (2, 2), st
+    assert st["kept_factor_bytes"] == 2 * (16 * npad * npad + 4 * npad), st
+    assert st["resident_media"] == 1, st
+
+
+def test_kept_factors_are_evicted_for_room(G):
+    """A budget whose half holds one factor: the second k evicts the first, so the first k
+    factors again; a log-det tile that needs the room drops the kept factor."""
+    med = G.Medium(3, 300, G.HARDSPHERE, (0.1,)).fill(22)
+    npad = (300 + 63) // 64 * 64
+    fbytes = 16 * npad * npad + 4 * npad
+    srv = G.Server(hbm_gb=3.0 * fbytes / 2 ** 30)
+    got = []
+    for k in (4.0 - 0.1j, 4.5 - 0.1j, 4.0 - 0.1j):
+        got.append(srv.wait(srv.submit_solve(med, k, G.PLANE, G.FP64))[0]["a"])
+    st = srv.stats()
+    assert (st["factor_hits"], st["factor_misses"]) == (0, 3), st
+    assert st["kept_factor_bytes"] == fbytes, st
+    ks = np.array([4.0 - 0.1j, 4.2 - 0.3j, 4.4 - 0.2j])
+    tl = srv.wait(srv.submit_logdet(med, ks, G.FP64))[0]["tracelog"]
+    st = srv.stats()
+    srv.close()
+    assert st["kept_factor_bytes"] == 0, st           # 3 x 16 N^2 did not fit beside it
+    assert st["peak_admitted_bytes"] <= st["budget_bytes"], st
+    want, _ = G.tracelog_batch(med, ks, G.FP64)
+    assert np.array_equal(tl, want)
+    for k, a in zip((4.0 - 0.1j, 4.5 - 0.1j, 4.0 - 0.1j), got):
+        assert np.array_equal(a, G.solve(med, k, G.PLANE, G.FP64)[0])
