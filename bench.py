@@ -610,7 +610,9 @@ def main():
     if prof32 is not None:
         g32 = prof32["gemm"]
         roofline_c64 = {"achieved": g32["flops"] / max(1e-12, g32["ms"] * 1e-3) / 1e12,
-                        "peak": PEAK_FP32_MFMA / 1e12, "frac": g32["flops"] / max(1e-12, g32["ms"] * 1e-3) / PEAK_FP32_MFMA}
+                        "peak": PEAK_FP32_MFMA / 1e12, "frac": g32["flops"] / max(1e-12, g32["ms"] * 1e-3) / PEAK_FP32_MFMA,
+                        "timing": "in the mix (four streams; each launch shares the GPU): compare with "
+                                  "roofline.frac_in_mix, not roofline.frac"}
     breakdown = {c: round(v["ms"] / max(1e-9, sum(x["ms"] for x in prof.values())), 4) for c, v in prof.items()}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(n, 48, 48)
