@@ -64,7 +64,7 @@ def measured_traffic(kernel):
     return None if rec is None else rec["read_bytes_per_launch"] + rec["write_bytes_per_launch"]
 
 
-def run_msmain(exe, n, nx, ny, threads):
+def run_msmain(exe, n, nx, ny, threads, extra_env=None):
     """Runs an msmain build on a d=3 unit-density k-map config; returns its realtime (s)."""
     tmp = tempfile.mkdtemp(prefix="msbench_")
     try:
@@ -74,6 +74,7 @@ def run_msmain(exe, n, nx, ny, threads):
                     "[kplane]\nnseed=1\nxrange=5:30\nyrange=-2:0\nxsample=%d\nysample=%d\n"
                     "color=temperature div 5\ntitle=bench\n" % (n, nx, ny))
         env = dict(os.environ, MKL_THREADING_LAYER="GNU", MKL_NUM_THREADS="1", OMP_NUM_THREADS=str(threads))
+        env.update(extra_env or {})
         subprocess.run([exe, "-p", str(threads), conf], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL, timeout=300)
         for dp, _, files in os.walk(os.path.join(tmp, "out")):
@@ -143,9 +144,12 @@ def dropin_msmain(n, nx, ny):
     threads = host_cores()
     try:
         rt = run_msmain(exe, n, nx, ny, threads)
+        rk = run_msmain(exe, n, nx, ny, threads, {"MSGPU_TRACELOG_KEEP_INPUT": "1"})
         return {"evals_per_s": nx * ny / rt, "threads": threads,
+                "evals_per_s_keep_input": nx * ny / rk,
                 "sample": "msmain_gpu -p %d: d=3 N=%d unit-density ball seed 1, hardsphere 0.05, %dx%d k-map "
-                          "= %d log-dets in %.2f s" % (threads, n, nx, ny, nx * ny, rt)}
+                          "= %d log-dets in %.2f s (%.2f s with MSGPU_TRACELOG_KEEP_INPUT=1: tracelog_lu leaves "
+                          "the matrix as given)" % (threads, n, nx, ny, nx * ny, rt, rk)}
     except Exception as ex:
         return {"evals_per_s": None, "threads": threads, "sample": "failed: %s" % ex}
 

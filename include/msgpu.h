@@ -96,7 +96,10 @@ void       build_normalized_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* 
 void       build_dk_ms_matrix(msgpu_medium* med, msgpu_cplx k, msgpu_cplx* matrix);
 void       build_plane_wave(msgpu_medium* med, msgpu_cplx k, double theta, msgpu_cplx* vec);
 /* tracelog_lu: callable from any number of threads at once; calls of the same n that
- * wait together share one batched factorization and each returns what it returns alone. */
+ * wait together share one batched factorization and each returns what it returns alone.
+ * The matrix is overwritten with zgetrf_'s LAPACK-format factor, as the reference's is;
+ * with the environment variable MSGPU_TRACELOG_KEEP_INPUT=1 it is left as given, which
+ * saves a third of the host-link traffic (the reference's callers never read it). */
 msgpu_cplx tracelog_lu(int n, msgpu_cplx* matrix);
 msgpu_cplx dk_log_det_ms(msgpu_medium* med, msgpu_cplx k);
 msgpu_cplx traceinv(int n, msgpu_cplx* matrix);

@@ -1631,6 +1631,17 @@ int lu_host(int n, cx<double>* a, int lda, int* ipiv, int nrhs, cx<double>* b, i
 // own factor.  Requests that arrive while batches run form the next ones; two batches
 // at a time let one's copies overlap the other's factorization.
 namespace {
+// MSGPU_TRACELOG_KEEP_INPUT=1: the drop-in tracelog_lu leaves the caller's matrix as given
+// instead of overwriting it with zgetrf_'s LAPACK-format factor (which the reference's
+// callers never read: ms_core.c:74 and 126 rebuild M before every call).  That saves the
+// factor's download and its unpacking: a third of the host-link bytes of a drop-in log-det.
+static bool tracelog_keeps_input() {
+    static const bool v = [] {
+        const char* e = getenv("MSGPU_TRACELOG_KEEP_INPUT");
+        return e && *e && *e != '0';
+    }();
+    return v;
+}
 struct TlReq {
     int n;
     const HostStage* hs;       // packed n x n matrix in, LAPACK-format factor out
@@ -1657,11 +1668,14 @@ void run_tl_batch(const std::vector<TlReq*>& B) {
     launch_pad_identity<double>(m.p, n, np, nb, L.st);
     factor<double>(m.p, np, nb, dpiv.p, dinfo.p, pmap.p, false, false, L.st);
     launch_tracelog<double>(m.p, n, nullptr, np, ms, dpiv.p, tl.p, nullptr, nb, L.st);
-    for (int k0 = NB; k0 < np; k0 += NB) launch_swap_left<double>(m.p, np, ms, pmap.p, k0, NB, nb, L.st);
+    const bool back = !tracelog_keeps_input();
+    if (back)
+        for (int k0 = NB; k0 < np; k0 += NB) launch_swap_left<double>(m.p, np, ms, pmap.p, k0, NB, nb, L.st);
     HIPCHK(hipGetLastError());
     std::vector<int> hpiv((size_t)nb * np), hinfo(nb);
     std::vector<cx<double>> htl(nb);
-    for (int i = 0; i < nb; i++) d2h_block_copy(m.p + (size_t)i * ms, np, n, n, *B[i]->hs, L.st);
+    if (back)
+        for (int i = 0; i < nb; i++) d2h_block_copy(m.p + (size_t)i * ms, np, n, n, *B[i]->hs, L.st);
     HIPCHK(hipMemcpyAsync(hpiv.data(), dpiv.p, sizeof(int) * (size_t)nb * np, hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipMemcpyAsync(hinfo.data(), dinfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipMemcpyAsync(htl.data(), tl.p, sizeof(cx<double>) * nb, hipMemcpyDeviceToHost, L.st));
@@ -1728,7 +1742,7 @@ int tracelog_host_coalesced(int n, cx<double>* a, int lda, int* ipiv, cx<double>
         }
         lk.unlock();
         if (!req.err.empty()) throw std::runtime_error(req.err);
-        unpack_block(a, lda, n, n, hs);
+        if (!tracelog_keeps_input()) unpack_block(a, lda, n, n, hs);
         *tracelog = req.tl;
         *info = req.info;
         return 0;

@@ -19,6 +19,7 @@ travel with the tree; the test fails (it does not skip) when they are missing.
 """
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -135,14 +136,15 @@ color2=temperature div 5
 """
 
 
-def run(exe, conf, tmp):
+def run(exe, conf, tmp, extra_env=None):
     assert os.path.exists(exe), "%s is missing: build it with `make -f oracle/ref.mk all gpu`" % exe
     os.makedirs(tmp, exist_ok=True)
     path = os.path.join(tmp, "t.conf")
     with open(path, "w") as f:
         f.write(conf)
     env = dict(os.environ, MKL_THREADING_LAYER="GNU", MKL_NUM_THREADS="1", OMP_NUM_THREADS="2")
-    print("running", os.path.basename(exe), "on", path, flush=True)
+    env.update(extra_env or {})
+    print("running", os.path.basename(exe), "on", path, extra_env or "", flush=True)
     subprocess.run([exe, "-p", "2", path], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL,
                    stderr=subprocess.PIPE, timeout=240)
     dats = [os.path.join(dp, fn) for dp, _, fs in os.walk(os.path.join(tmp, "out")) for fn in fs if fn.endswith(".dat")]
@@ -193,6 +195,37 @@ def test_tracelog_tasks(outputs, section):
         print("[%s] %d cells: max rel |d Re| %.2e, max |d arg det| %.2e" % (section, a.size, ere, earg))
         assert ere <= 1e-10
         assert earg <= 1e-10
+
+
+def test_tracelog_keep_input_same_map(tmp_path):
+    """MSGPU_TRACELOG_KEEP_INPUT=1 (tracelog_lu leaves the matrix as given instead of writing
+    zgetrf_'s factor back) changes no trace-log that msmain writes."""
+    plain = run(GPU_EXE, CONF_D3, str(tmp_path / "plain"))
+    keep = run(GPU_EXE, CONF_D3, str(tmp_path / "keep"), {"MSGPU_TRACELOG_KEEP_INPUT": "1"})
+    for sp, sk in zip(plain["kplane"], keep["kplane"]):
+        assert sp["data"] == sk["data"]
+
+
+def test_tracelog_keep_input_leaves_matrix(tmp_path):
+    """In a process with MSGPU_TRACELOG_KEEP_INPUT=1, tracelog_lu returns the same value and
+    leaves its argument unchanged."""
+    code = (
+        "import numpy as np, msmodel_amd as G\n"
+        "rng = np.random.default_rng(3)\n"
+        "m = rng.normal(size=(90, 90)) + 1j * rng.normal(size=(90, 90))\n"
+        "t, a = G.tracelog_lu(m)\n"
+        "assert np.array_equal(a, m)\n"
+        "np.save(%r, np.array([t]))\n" % str(tmp_path / "t.npy"))
+    env = dict(os.environ, MSGPU_TRACELOG_KEEP_INPUT="1",
+               PYTHONPATH=os.pathsep.join([ROOT] + os.environ.get("PYTHONPATH", "").split(os.pathsep)))
+    print("running a python process with MSGPU_TRACELOG_KEEP_INPUT=1", flush=True)
+    subprocess.run([sys.executable, "-c", code], env=env, check=True, timeout=240)
+    import msmodel_amd as G
+    rng = np.random.default_rng(3)
+    m = rng.normal(size=(90, 90)) + 1j * rng.normal(size=(90, 90))
+    t, a = G.tracelog_lu(m)
+    assert not np.array_equal(a, m)            # the default writes the factor back
+    assert np.load(str(tmp_path / "t.npy"))[0] == t
 
 
 def test_wavefunction(outputs):
