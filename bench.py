@@ -10,6 +10,10 @@ get_arg_cmap).  Positions and the k grid are uploaded once per call (24 KB +
 
 Paths timed on the same map and GPU:
   fp64 batched   complex128 assembly + blocked LU on MFMA, all cells batched
+  fp64 ldlt      complex128, pivot-free LDL^T of the symmetric M (half the update
+                 flops, no pivot search) on the cells whose growth is predicted
+                 small, the blocked LU on the others and on any cell whose measured
+                 growth max |L| exceeds 1e3
   mixed          complex64 LU on MFMA for the whole map, complex128 re-evaluation
                  of cells with |det| < 1e-6 max|det| (the near-zero cells)
   fp64 plain     one matrix per call in complex128 (the plain FP64 GPU path)
@@ -374,19 +378,19 @@ def small_map(G, steps, tune):
             "launch_shapes": tuned}
 
 
-def serialized_pass(G, med, ks, np_):
-    """One pass of the map on a single stream (the tuning entry of np_ with streams=1),
-    every launch bracketed by HIP events on that stream: the kernels run one after the
-    other, so each launch's event interval is its duration alone on the GPU."""
+def serialized_pass(G, med, ks, np_, mode):
+    """One pass of the map in the given mode on a single stream (the tuning entry of np_
+    with streams=1), every launch bracketed by HIP events on that stream: the kernels run
+    one after the other, so each launch's event interval is its duration alone on the GPU."""
     tuned = tune_entry(G, np_, 0)
     if tuned is None:
         return None
     set_entry(G, np_, 0, dict(tuned, streams=1))
     try:
-        G.tracelog_batch(med, ks, G.FP64)
+        G.tracelog_batch(med, ks, mode)
         G.profile_begin()
         t0 = time.perf_counter()
-        G.tracelog_batch(med, ks, G.FP64)
+        G.tracelog_batch(med, ks, mode)
         wall = time.perf_counter() - t0
         prof = G.profile_end()
     finally:
@@ -445,6 +449,8 @@ def main():
     ap.add_argument("--no-small", action="store_true", help="skip the N=200 map (eager vs HIP graph)")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 path only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-path", default="fp64_ldlt", choices=("fp64_batched", "fp64_ldlt"),
+                    help="the path whose serialized pass --roofline-only runs")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the serialized pass of the map (one stream), for the rocprof kernel-trace run")
     ap.add_argument("--no-dropin", action="store_true", help="skip timing msmain relinked against the library")
@@ -494,7 +500,8 @@ def main():
 
     if args.roofline_only:
         np_ = (n + 63) // 64 * 64
-        prof, wall = serialized_pass(G, med, ks, np_)     # its warm-up pass is serialized too
+        mode = {"fp64_batched": G.FP64, "fp64_ldlt": G.FP64_LDLT}[args.roofline_path]
+        prof, wall = serialized_pass(G, med, ks, np_, mode)     # its warm-up pass is serialized too
         g = prof["gemm"]
         print(json.dumps({"serialized_pass_ms": 1e3 * wall, "gemm_launches": g["launches"],
                           "gemm_avg_ms": g["ms"] / max(1, g["launches"]),
@@ -516,6 +523,9 @@ def main():
     out64, _, t64, prof64 = timed_path(G.FP64)
     progress("fp64 map: %.1f evals/s" % (args.steps * nk / t64))
     fp64_batched = args.steps * nk / t64
+    outL, flL, tL, profL = timed_path(G.FP64_LDLT)
+    ldlt_rate = args.steps * nk / tL
+    progress("fp64 ldlt map: %.1f evals/s, %d cells through the LU" % (ldlt_rate, int(flL.sum())))
     if args.no_mixed:   # (profiler runs of the FP64 path alone)
         out_mixed, flags, t_mixed, prof32 = out64, np.zeros(nk, np.int32), t64, None
         mixed_rate = None
@@ -575,27 +585,34 @@ def main():
         progress("N=200 map done")
 
     mixed_ok = float(dtl.max()) <= 1e-8
-    if mixed_rate is not None and mixed_ok and mixed_rate > fp64_batched:
-        value, value_path, prof, peak, kname = mixed_rate, "mixed", prof32, PEAK_FP32_MFMA, "gemm_sub_kernel<float>"
-        ms_step = 1e3 * t_mixed / args.steps
-    else:
-        value, value_path, prof, peak, kname = fp64_batched, "fp64_batched", prof64, PEAK_FP64_MFMA, \
-            "gemm_sub_kernel<double>"
-        ms_step = 1e3 * t64 / args.steps
+    dtl_ldlt, dre_ldlt = dtracelog(outL)
+    ldlt_ok = float(dtl_ldlt.max()) <= 1e-8
+    # the fastest batched path within 1e-8 of the complex128 LU map
+    cands = [(fp64_batched, "fp64_batched", prof64, PEAK_FP64_MFMA, "gemm_sub_kernel<double>", t64, G.FP64)]
+    if ldlt_ok:
+        cands.append((ldlt_rate, "fp64_ldlt", profL, PEAK_FP64_MFMA, "gemm_sub_kernel<double>", tL, G.FP64_LDLT))
+    if mixed_rate is not None and mixed_ok:
+        cands.append((mixed_rate, "mixed", prof32, PEAK_FP32_MFMA, "gemm_sub_kernel<float>", t_mixed, G.MIXED))
+    value, value_path, prof, peak, kname, t_value, value_mode = max(cands, key=lambda c: c[0])
+    ms_step = 1e3 * t_value / args.steps
     # The dominant kernel's roofline from a serialized pass (one stream: each launch alone on
     # the GPU, durations from HIP events on its stream), measured here; the same launches in
     # the timed region (four streams sharing the GPU, which stretches each launch's event
     # interval) are reported beside it as *_in_mix.
     g = prof["gemm"]
-    ser = serialized_pass(G, med, ks, (n + 63) // 64 * 64) if value_path == "fp64_batched" else None
+    ser = serialized_pass(G, med, ks, (n + 63) // 64 * 64, value_mode) if value_path != "mixed" else None
     gs = ser[0]["gemm"] if ser else g
     achieved = gs["flops"] / (gs["ms"] * 1e-3) if gs["ms"] > 0 else 0.0
     achieved_mix = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
+    lu_counters = value_path == "fp64_batched"      # the recorded counter runs are of the LU map's GEMM
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": measured_traffic(kname),
+                "frac": achieved / peak, "traffic": measured_traffic(kname) if lu_counters else None,
                 "traffic_source": "profiles/traffic.json, from the counter runs of profiles/pmc_lu_map_n1000.md "
-                                  "(rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE per launch, scaled by a 1 GiB copy)",
-                "kernel": kname + " (LU trailing update, MFMA 16x16x4)",
+                                  "(rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE per launch, scaled by a 1 GiB copy)"
+                                  if lu_counters else "no counter run of this path's launches is recorded",
+                "kernel": kname + (" (trailing updates of the LDL^T (lower tiles) and of the LU, MFMA 16x16x4)"
+                                   if value_path == "fp64_ldlt" else " (LU trailing update, MFMA 16x16x4)"),
+                "path": value_path,
                 "timing": "serialized pass: one stream, HIP events around every launch" if ser else
                           "timed region (four streams)",
                 "launches": gs["launches"], "avg_ms": gs["ms"] / max(1, gs["launches"]),
@@ -632,12 +649,15 @@ def main():
         "metric": "log-det evals/s at N=1000 (mixed vs FP64, same GPU); p50/p99 resonance latency",
         "value": value, "unit": "log-det evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c128" if value_path == "fp64_batched" else "c64+c128", "data": "synthetic",
+        "vs_baseline": None, "dtype": "c64+c128" if value_path == "mixed" else "c128", "data": "synthetic",
         "config": {"workload": "tracelog map d=3 N=%d unit-density ball seed 1 hardsphere 0.05, %dx%d k over "
                                "Re[5,30] x Im[-2,0]" % (n, args.grid, args.grid),
                    "cells_per_step": nk},
         "value_path": value_path,
         "fp64_batched_evals_per_s": fp64_batched, "mixed_evals_per_s": mixed_rate,
+        "fp64_ldlt": {"evals_per_s": ldlt_rate, "lu_cells": int(flL.sum()), "cells": nk,
+                      "max_abs_dtracelog": float(dtl_ldlt.max()), "max_abs_dlogdet": float(n * dtl_ldlt.max()),
+                      "max_abs_dtracelog_re": float(dre_ldlt.max())},
         "fp64_plain_evals_per_s": fp64_plain, "value_over_fp64_plain": value / fp64_plain if fp64_plain else None,
         "mixed_reevaluated_cells": int(flags.sum()), "mixed_reeval_rel": 1e-6,
         "mixed_max_abs_dtracelog": float(dtl.max()), "mixed_median_abs_dtracelog": float(np.median(dtl)),

@@ -75,7 +75,10 @@ enum {
     MSGPU_FP64 = 0,    /* complex128 everywhere                                      */
     MSGPU_MIXED = 1,   /* complex64 LU on MFMA; FP64 re-evaluation of flagged cells  */
     MSGPU_FP32 = 2,    /* complex64 LU only (error reported, not corrected)          */
-    MSGPU_MIXED_GATED = 3  /* complex64 LU; FP64 where its pivot growth could exceed tol */
+    MSGPU_MIXED_GATED = 3, /* complex64 LU; FP64 where its pivot growth could exceed tol */
+    MSGPU_FP64_LDLT = 4    /* complex128; pivot-free LDL^T where its growth stays small,
+                            * pivoted LU elsewhere (trace-log batches; complex128 in the
+                            * other entry points)                                      */
 };
 
 /* ------------------------------------------------- 1. drop-in (reference ABI) */

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libmsgpu.so")
 HARDSPHERE, SOFTSPHERE, RESONANT, MAXIMUM = 0, 1, 2, 3
 CUBE, BALL, LATTICE, GAUSSIAN = 0, 1, 2, 3
 PLANE, SPHERICAL = 0, 1
-FP64, MIXED, FP32, MIXED_GATED = 0, 1, 2, 3
+FP64, MIXED, FP32, MIXED_GATED, FP64_LDLT = 0, 1, 2, 3, 4
 
 if not os.path.exists(LIB_PATH):
     raise ImportError("msmodel_amd: %s is not built; run `python msmodel_amd/build.py` "

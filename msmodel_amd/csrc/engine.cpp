@@ -20,6 +20,7 @@
 #include <string>
 #include <thread>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "capi_internal.h"
@@ -809,14 +810,49 @@ static ChunkGraph<T>* chunk_graph(hipStream_t st, int np, int B, bool rm, const 
     return raw;
 }
 
+// Pivot-free LDL^T of nbatch symmetric matrices (kern_ldlt.hip) in 128-column blocks:
+// the block's first 64 columns (diagonal block, rows), their update of its second 64
+// columns, the second 64 columns, then one K = 128 update of the lower tiles right of
+// the block (the complex128 GEMM's 3M product).  growth[b] receives the largest |L| of
+// matrix b (as the bits of a non-negative double).
+static void factor_ldlt(cx<double>* mats, int np, int nbatch, unsigned long long* growth, hipStream_t st) {
+    const long long ms = (long long)np * np;
+    auto at = [&](int r, int c) { return mats + r + (size_t)c * np; };
+    auto panel = [&](int k0) {
+        timed(0, 0.0, 0.0, st, [&] { launch_ldlt_diag<double>(mats, np, ms, k0, growth, nbatch, st); });
+        if (np - k0 - NB > 0)
+            timed(1, 0.0, 0.0, st, [&] { launch_ldlt_rows<double>(mats, np, ms, k0, growth, nbatch, st); });
+    };
+    HIPCHK(hipMemsetAsync(growth, 0, sizeof(unsigned long long) * nbatch, st));
+    for (int k0 = 0; k0 < np; k0 += 2 * NB) {
+        panel(k0);
+        const int W1 = np - k0 - NB;
+        if (W1 <= 0) break;
+        timed(2, 8.0 * W1 * NB * NB * nbatch, 0.0, st, [&] {
+            launch_gemm_sub<double>(at(k0 + NB, k0 + NB), at(k0 + NB, k0), at(k0, k0 + NB), W1, NB, NB, np, np, np, ms,
+                                    ms, ms, nbatch, st);
+        });
+        panel(k0 + NB);
+        const int W2 = np - k0 - 2 * NB;
+        if (W2 <= 0) break;
+        timed(2, 8.0 * W2 * (W2 + (double)NB) / 2.0 * (2 * NB) * nbatch, 0.0, st, [&] {
+            launch_gemm_sub_lower<double>(at(k0 + 2 * NB, k0 + 2 * NB), at(k0 + 2 * NB, k0), at(k0, k0 + 2 * NB), W2,
+                                          2 * NB, np, np, np, ms, ms, ms, nbatch, st);
+        });
+    }
+}
+
+// ldlt (complex128 only): the pivot-free LDL^T instead of the LU; stats[2 l] receives the
+// growth max |L| of cell l (stats[2 l + 1] = 0).
 template <class T>
 static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
-                          cx<double>* out, double* stats) {
+                          cx<double>* out, double* stats, bool ldlt = false) {
     Lanes S(streams_for(max_np(meds, nk, midx), prec_of<T>()));
     const Plan P = make_plan(meds, nk, midx, sizeof(cx<T>), 1, S.ns);
     DeviceItems D(meds, nk, midx, ks, P.order, S[0]);
     DBuf<cx<double>> dout(nk);
     DBuf<double> dstats(2 * (size_t)nk);
+    DBuf<unsigned long long> dgrow(ldlt ? nk : 1);
     S.fork();
     const int nlane = std::min<int>(S.ns, (int)P.chunks.size());
     std::vector<DBuf<cx<T>>> mats(nlane);
@@ -837,6 +873,19 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
         hipStream_t st = S[ln];
         if (c > 0 && (int)c < nlane) HIPCHK(hipStreamWaitEvent(st, stag[c - 1], 0));
         const Tune tn = tune_for(np, prec_of<T>());
+        if constexpr (std::is_same<T, double>::value) {
+            if (ldlt) {
+                if ((int)c + 1 < nlane) HIPCHK(hipEventRecord(stag[c], st));
+                timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st,
+                      [&] { launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st); });
+                factor_ldlt(mats[ln].p, np, B, dgrow.p + ch.start, st);
+                timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {
+                    launch_tracelog<T>(mats[ln].p, 0, D.items.p + ch.start, np, ms, nullptr, dout.p + ch.start,
+                                       nullptr, B, st);
+                });
+                continue;
+            }
+        }
         ChunkGraph<T>* cg = (tn.graph && !g_prof_on) ? chunk_graph<T>(st, np, B, tn.rm == 1, tn) : nullptr;
         if (cg) {
             HIPCHK(hipMemcpyAsync(cg->items.p, D.items.p + ch.start, sizeof(MatItem) * B, hipMemcpyDeviceToDevice, st));
@@ -859,9 +908,19 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
     S.join();
     std::vector<cx<double>> hout(nk);
     std::vector<double> hst(stats ? 2 * (size_t)nk : 0);
+    std::vector<unsigned long long> hgrow(ldlt && stats ? nk : 0);
     HIPCHK(hipMemcpyAsync(hout.data(), dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, S[0]));
-    if (stats) HIPCHK(hipMemcpyAsync(hst.data(), dstats.p, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, S[0]));
+    if (stats && !ldlt)
+        HIPCHK(hipMemcpyAsync(hst.data(), dstats.p, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, S[0]));
+    if (stats && ldlt)
+        HIPCHK(hipMemcpyAsync(hgrow.data(), dgrow.p, sizeof(unsigned long long) * nk, hipMemcpyDeviceToHost, S[0]));
     S.l[0].sync();
+    for (size_t q = 0; q < hgrow.size(); q++) {
+        double g;
+        memcpy(&g, &hgrow[q], sizeof g);
+        hst[2 * q] = g;
+        hst[2 * q + 1] = 0.0;
+    }
     for (hipEvent_t e : stag)
         if (e) (void)hipEventDestroy(e);
     for (int q = 0; q < nk; q++) {
@@ -889,7 +948,7 @@ int tracelog_multi(const std::vector<MediumView>& meds, int nk, const int* midx,
     if (int rc = check_media(meds, nk, midx)) return rc;
     return guarded([&]() -> int {
         if (precision == 1) tracelog_impl<float>(meds, nk, midx, ks, out, stats);
-        else tracelog_impl<double>(meds, nk, midx, ks, out, stats);
+        else tracelog_impl<double>(meds, nk, midx, ks, out, stats, precision == 2);
         return 0;
     });
 }

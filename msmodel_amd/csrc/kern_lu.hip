@@ -221,9 +221,10 @@ constexpr int KC3 = 8, K3M = 128;
 template <class T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void gemm_sub_kernel(
     cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int kk, int lda, int ldb, int ldc, long long sC,
-    long long sA, long long sB, int tiles_m, int tiles_n, int nbatch) {
+    long long sA, long long sB, int tiles_m, int tiles_n, int nbatch, int lower) {
     __shared__ __align__(16) T lds[KC * (2 * GT + PADA) + GT * (2 * KC + PADB)];
-    const int tiles = tiles_m * tiles_n;
+    // lower != 0: only the tiles on and below the diagonal of a square C (tiles_m == tiles_n)
+    const int tiles = lower ? tiles_m * (tiles_m + 1) / 2 : tiles_m * tiles_n;
     int g = blockIdx.x, b, t;
     if ((nbatch & 7) == 0) {   // keep every tile of a matrix on one XCD (dispatch is round-robin over 8)
         const int xcd = g & 7, j = g >> 3;
@@ -233,7 +234,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
         b = g / tiles;
         t = g % tiles;
     }
-    const int row0 = (t % tiles_m) * GT, col0 = (t / tiles_m) * GT;
+    int ti = t % tiles_m, tj = t / tiles_m;
+    if (lower) {            // column tj holds tiles_m - tj tiles, rows tj .. tiles_m - 1
+        tj = 0;
+        while (t >= tiles_m - tj) { t -= tiles_m - tj; tj++; }
+        ti = tj + t;
+    }
+    const int row0 = ti * GT, col0 = tj * GT;
     C += (size_t)b * sC; A += (size_t)b * sA; B += (size_t)b * sB;
     if constexpr (sizeof(T) == 8) {
         if (kk >= K3M) {
@@ -257,7 +264,7 @@ __global__ __launch_bounds__(256) void tracelog_kernel(const cx<T>* __restrict__
     const int b = blockIdx.x, tid = threadIdx.x;
     if (items) n = items[b].n;
     const cx<T>* M = A + (size_t)b * mstride;
-    const int* ipiv = ipiv_all + (size_t)b * np;
+    const int* ipiv = ipiv_all ? ipiv_all + (size_t)b * np : nullptr;      // null: no interchanges
     double re = 0.0, im = 0.0, mx = -INFINITY, mn = INFINITY;
     int odd = 0;
     for (int i = tid; i < n; i += 256) {
@@ -267,7 +274,7 @@ __global__ __launch_bounds__(256) void tracelog_kernel(const cx<T>* __restrict__
         im += atan2((double)u.y, (double)u.x);
         mx = fmax(mx, la);
         mn = fmin(mn, la);
-        odd ^= (ipiv[i] != i + 1);
+        if (ipiv) odd ^= (ipiv[i] != i + 1);
     }
     sre[tid] = re; sim[tid] = im; smax[tid] = mx; smin[tid] = mn; sodd[tid] = odd;
     __syncthreads();
@@ -445,7 +452,16 @@ void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int
     if (m <= 0 || n <= 0 || k <= 0) return;
     const int tm = (m + GT - 1) / GT, tn = (n + GT - 1) / GT;
     hipLaunchKernelGGL(gemm_sub_kernel<T>, dim3(tm * tn * nbatch), dim3(256), 0, st, C, A, B, m, n, k, lda, ldb, ldc,
-                       sC, sA, sB, tm, tn, nbatch);
+                       sC, sA, sB, tm, tn, nbatch, 0);
+}
+
+template <class T>
+void launch_gemm_sub_lower(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int k, int lda, int ldb, int ldc,
+                           long long sC, long long sA, long long sB, int nbatch, hipStream_t st) {
+    if (m <= 0 || k <= 0) return;
+    const int tm = (m + GT - 1) / GT;
+    hipLaunchKernelGGL(gemm_sub_kernel<T>, dim3(tm * (tm + 1) / 2 * nbatch), dim3(256), 0, st, C, A, B, m, m, k, lda,
+                       ldb, ldc, sC, sA, sB, tm, tm, nbatch, 1);
 }
 
 template <class T>
@@ -641,6 +657,8 @@ void launch_panel(cx<T>* A, int np, long long mstride, int* ipiv, int* info, int
                                   int, int);                                                                       \
     template void launch_gemm_sub<T>(cx<T>*, const cx<T>*, const cx<T>*, int, int, int, int, int, int, long long,      \
                                      long long, long long, int, hipStream_t);                                          \
+    template void launch_gemm_sub_lower<T>(cx<T>*, const cx<T>*, const cx<T>*, int, int, int, int, int, long long,    \
+                                           long long, long long, int, hipStream_t);                                    \
     template void launch_tracelog<T>(const cx<T>*, int, const MatItem*, int, long long, const int*, cx<double>*,     \
                                      double*, int, hipStream_t);                                                       \
     template void launch_gather_rows<T>(cx<T>*, const cx<T>*, const int*, int, int, long long, long long, int,       \

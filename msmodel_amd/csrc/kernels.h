@@ -80,7 +80,20 @@ void launch_swap_range(cx<T>* A, int np, long long mstride, const int* pmap, int
 template <class T>
 void launch_gemm_sub(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int n, int k, int lda, int ldb, int ldc,
                      long long sC, long long sA, long long sB, int nbatch, hipStream_t st);
-// n of matrix b is items[b].n when items is given, else n
+// C -= A B on the tiles on and below the diagonal of the square m x m C only (the
+// trailing update of the symmetric factorisation)
+template <class T>
+void launch_gemm_sub_lower(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int k, int lda, int ldb, int ldc,
+                           long long sC, long long sA, long long sB, int nbatch, hipStream_t st);
+// Pivot-free LDL^T of symmetric matrices (kern_ldlt.hip), column-major, ld np: the 64 x 64
+// diagonal block at k0 becomes L11 (unit lower) and U11 = D11 L11^T; then the rows below
+// get L21 = A21 U11^-1 and the rows k0 .. k0+63 right of the block U12 = D11 L21^T.
+// growth[b] (bits of a non-negative double) = the largest |L| seen.
+template <class T>
+void launch_ldlt_diag(cx<T>* A, int np, long long ms, int k0, unsigned long long* growth, int nbatch, hipStream_t st);
+template <class T>
+void launch_ldlt_rows(cx<T>* A, int np, long long ms, int k0, unsigned long long* growth, int nbatch, hipStream_t st);
+// n of matrix b is items[b].n when items is given, else n; ipiv null: no interchanges
 template <class T>
 void launch_tracelog(const cx<T>* A, int n, const MatItem* items, int np, long long mstride, const int* ipiv,
                      cx<double>* out, double* stats, int nbatch, hipStream_t st);
