@@ -754,6 +754,13 @@ struct DeviceItems {
         HIPCHK(hipMemcpyAsync(items.p, hitems.data(), sizeof(MatItem) * nk, hipMemcpyHostToDevice, st));
         launch_item_diag(items.p, nk, st);
     }
+    // the dimension d shared by the items [s, s + B) in plan order, 0 when they differ
+    int dim(int s, int B) const {
+        const int d = hitems[s].d;
+        for (int q = s + 1; q < s + B; q++)
+            if (hitems[q].d != d) return 0;
+        return d;
+    }
 };
 
 // ------------------------------------------------------------------ HIP graphs
@@ -777,10 +784,10 @@ static std::mutex g_graph_mu;
 static size_t g_graph_bytes = 0;
 
 template <class T>
-static ChunkGraph<T>* chunk_graph(hipStream_t st, int np, int B, bool rm, const Tune& tn) {
-    using Key = std::tuple<hipStream_t, int, int, int, int, int, int>;
+static ChunkGraph<T>* chunk_graph(hipStream_t st, int np, int B, bool rm, const Tune& tn, int dall) {
+    using Key = std::tuple<hipStream_t, int, int, int, int, int, int, int>;
     static std::map<Key, std::unique_ptr<ChunkGraph<T>>> cache;
-    const Key key{st, np, B, rm ? 1 : 0, tn.swap, tn.panel, tn.blk2};
+    const Key key{st, np, B, rm ? 1 : 0, tn.swap, tn.panel, tn.blk2, dall == 3 ? 3 : 0};
     std::lock_guard<std::mutex> g(g_graph_mu);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second.get();
@@ -796,7 +803,7 @@ static ChunkGraph<T>* chunk_graph(hipStream_t st, int np, int B, bool rm, const 
     cg->out.alloc(B);
     cg->stats.alloc(2 * (size_t)B);
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    launch_assemble_items<T>(cg->items.p, np, 0, cg->mats.p, ms, B, st);
+    launch_assemble_items<T>(cg->items.p, np, 0, cg->mats.p, ms, B, st, dall);
     factor<T>(cg->mats.p, np, B, cg->ipiv.p, cg->info.p, cg->pmap.p, false, rm, st);
     launch_tracelog<T>(cg->mats.p, 0, cg->items.p, np, ms, cg->ipiv.p, cg->out.p, cg->stats.p, B, st);
     hipGraph_t graph = nullptr;
@@ -876,8 +883,9 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
         if constexpr (std::is_same<T, double>::value) {
             if (ldlt) {
                 if ((int)c + 1 < nlane) HIPCHK(hipEventRecord(stag[c], st));
-                timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st,
-                      [&] { launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st); });
+                timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st, [&] {
+                    launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st, D.dim(ch.start, B));
+                });
                 factor_ldlt(mats[ln].p, np, B, dgrow.p + ch.start, st);
                 timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {
                     launch_tracelog<T>(mats[ln].p, 0, D.items.p + ch.start, np, ms, nullptr, dout.p + ch.start,
@@ -886,7 +894,8 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
                 continue;
             }
         }
-        ChunkGraph<T>* cg = (tn.graph && !g_prof_on) ? chunk_graph<T>(st, np, B, tn.rm == 1, tn) : nullptr;
+        const int dall = D.dim(ch.start, B);
+        ChunkGraph<T>* cg = (tn.graph && !g_prof_on) ? chunk_graph<T>(st, np, B, tn.rm == 1, tn, dall) : nullptr;
         if (cg) {
             HIPCHK(hipMemcpyAsync(cg->items.p, D.items.p + ch.start, sizeof(MatItem) * B, hipMemcpyDeviceToDevice, st));
             if ((int)c + 1 < nlane) HIPCHK(hipEventRecord(stag[c], st));
@@ -897,7 +906,7 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
             continue;
         }
         timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st,
-              [&] { launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st); });
+              [&] { launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st, dall); });
         factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, false, tn.rm == 1, st,
                   ((int)c + 1 < nlane) ? stag[c] : nullptr);
         timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {
@@ -993,7 +1002,7 @@ static void dk_impl(const std::vector<MediumView>& meds, int nk, const int* midx
         // M^-1 = P V with V the solve of the row-major factor against the identity; then
         //   tr(M^-1)    = sum_i V[i, perm[i]]                 (traceinv)
         //   tr(M^-1 M') = sum_i sum_c V[i, c] M'[perm[i], c]  (M' generated on the fly)
-        launch_assemble_items<T>(it, np, 0, mats[ln].p, ms, B, st);
+        launch_assemble_items<T>(it, np, 0, mats[ln].p, ms, B, st, D.dim(ch.start, B));
         HIPCHK(launch_set_identity<T>(rhs[ln].p, np, ms, B, st));
         factor<T>(mats[ln].p, np, B, ipiv[ln].p, info[ln].p, pmap[ln].p, true, true, st);
         lu_solve_rm<T>(mats[ln].p, rhs[ln].p, np, np, B, ms, st, true);
@@ -1335,7 +1344,7 @@ int amplitudes_multi(const std::vector<MediumView>& meds, int nk, const int* mid
         for (const Plan::Chunk& ch : P.chunks) {
             const int B = ch.count, np = ch.np;
             const long long ms = (long long)np * np;
-            launch_assemble_items<double>(D.items.p + ch.start, np, 0, mats.p, ms, B, L.st);
+            launch_assemble_items<double>(D.items.p + ch.start, np, 0, mats.p, ms, B, L.st, D.dim(ch.start, B));
             factor<double>(mats.p, np, B, ipiv.p, info.p, pmap.p, true, true, L.st);
             launch_build_perm(pmap.p, np, NB, perm.p, B, L.st);
             launch_plane_items(D.items.p + ch.start, np, X.p, B, L.st);

@@ -71,10 +71,15 @@ __device__ __forceinline__ cx<T> neg_green3_s(cx<T> k, T s) {
 // Green function once per unordered pair and writes both (I,J) and, through an
 // LDS transpose, (J,I), so M = M^T exactly (ms_matrix_util.c:36-41 mirrors the
 // upper triangle the same way).
+//
+// D3: every matrix of the launch is d = 3 and kind 0.  The general kernel carries the
+// Bessel-function code of d = 2 and 4 and needs 208 VGPRs (two waves per SIMD), which
+// leaves the d = 3 stores little to overlap with; the specialised one has only the d = 3
+// Green function.
 constexpr int AT = 32;                 // tile edge (256 threads: AT / (256 / AT) elements each)
 constexpr int ARW = 256 / AT;           // thread rows of a tile
 
-template <class T>
+template <class T, bool D3>
 __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict__ pos, int n, int d, int np,
                                                        const cx<double>* __restrict__ ks,
                                                        const cx<double>* __restrict__ diag,
@@ -122,7 +127,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
             v = cx<T>{gi == gj ? T(1) : T(0), T(0)};
         } else if (gi == gj) {
             v = cx<T>{(T)dg.x, (T)dg.y};
-        } else if (d == 3 && kind == 0) {
+        } else if (D3 || (d == 3 && kind == 0)) {
             double s = 0.0;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -130,7 +135,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
                 s += e * e;
             }
             v = neg_green3_s<T, false>(k, (T)s);
-        } else {
+        } else if constexpr (!D3) {
             double s = 0.0;
             for (int c = 0; c < d; c++) {
                 double e = sx[0][ii][c] - sx[1][jj][c];
@@ -164,21 +169,30 @@ void launch_assemble(const double* pos, int n, int d, int np, const cx<double>* 
                      const cx<double>* izero, int kind, cx<T>* out, long long mstride, int nbatch, hipStream_t st) {
     const int nt = np / AT;
     dim3 grid(nt * (nt + 1) / 2, nbatch);
-    hipLaunchKernelGGL(assemble_kernel<T>, grid, dim3(256), 0, st, pos, n, d, np, ks, diag, izero, kind, out,
-                       mstride, (const MatItem*)nullptr);
+    if (d == 3 && kind == 0)
+        hipLaunchKernelGGL((assemble_kernel<T, true>), grid, dim3(256), 0, st, pos, n, d, np, ks, diag, izero, kind,
+                           out, mstride, (const MatItem*)nullptr);
+    else
+        hipLaunchKernelGGL((assemble_kernel<T, false>), grid, dim3(256), 0, st, pos, n, d, np, ks, diag, izero, kind,
+                           out, mstride, (const MatItem*)nullptr);
 }
 
 template <class T>
 void launch_assemble_items(const MatItem* items, int np, int kind, cx<T>* out, long long mstride, int nbatch,
-                           hipStream_t st) {
+                           hipStream_t st, int dall) {
     const int nt = np / AT;
     dim3 grid(nt * (nt + 1) / 2, nbatch);
-    hipLaunchKernelGGL(assemble_kernel<T>, grid, dim3(256), 0, st, (const double*)nullptr, 0, 0, np,
-                       (const cx<double>*)nullptr, (const cx<double>*)nullptr, (const cx<double>*)nullptr, kind, out,
-                       mstride, items);
+    if (dall == 3 && kind == 0)
+        hipLaunchKernelGGL((assemble_kernel<T, true>), grid, dim3(256), 0, st, (const double*)nullptr, 0, 0, np,
+                           (const cx<double>*)nullptr, (const cx<double>*)nullptr, (const cx<double>*)nullptr, kind,
+                           out, mstride, items);
+    else
+        hipLaunchKernelGGL((assemble_kernel<T, false>), grid, dim3(256), 0, st, (const double*)nullptr, 0, 0, np,
+                           (const cx<double>*)nullptr, (const cx<double>*)nullptr, (const cx<double>*)nullptr, kind,
+                           out, mstride, items);
 }
-template void launch_assemble_items<double>(const MatItem*, int, int, cx<double>*, long long, int, hipStream_t);
-template void launch_assemble_items<float>(const MatItem*, int, int, cx<float>*, long long, int, hipStream_t);
+template void launch_assemble_items<double>(const MatItem*, int, int, cx<double>*, long long, int, hipStream_t, int);
+template void launch_assemble_items<float>(const MatItem*, int, int, cx<float>*, long long, int, hipStream_t, int);
 template void launch_assemble<double>(const double*, int, int, int, const cx<double>*, const cx<double>*,
                                       const cx<double>*, int, cx<double>*, long long, int, hipStream_t);
 template void launch_assemble<float>(const double*, int, int, int, const cx<double>*, const cx<double>*,

@@ -127,7 +127,7 @@ void run_map(unsigned char* base, const MediumView& mv, int np, int B, const Lay
         hipLaunchKernelGGL(noalloc_items_kernel, dim3((b + 63) / 64), dim3(64), 0, st, items, dks, dpos, mv.n,
                            mv.d, mv.m, b);
         launch_item_diag(items, b, st);
-        launch_assemble_items<T>(items, np, 0, mats, ms, b, st);
+        launch_assemble_items<T>(items, np, 0, mats, ms, b, st, mv.d);
         factor_untimed<T>(mats, np, b, ipiv, info, ws, st);
         launch_tracelog<T>(mats, 0, items, np, ms, ipiv, dout, nullptr, b, st);
         NACHK(hipGetLastError());
