@@ -18,7 +18,8 @@ Paths timed on the same map and GPU:
                  of cells with |det| < 1e-6 max|det| (the near-zero cells)
   fp64 plain     one matrix per call in complex128 (the plain FP64 GPU path)
 value is the throughput of the fastest batched path whose error against the
-complex128 map meets |d tracelog| <= 1e-8 (value_path names it).
+complex128 LU map meets |d log det| <= 1e-8 in every cell (log det = N tracelog;
+value_path names it).
 Also reported: a Poisson stream of mixed requests (log-det tiles, solves with a psi
 grid, resonance searches) through the serving loop with p50/p99 latency, a
 resonance search over 64 media of uneven N in one call, the N=1000 solve and
@@ -52,19 +53,20 @@ def kgrid(nx, ny, xmin, xmax, ymin, ymax):
     return (xmin + (l % nx) * dx) + 1j * (ymax - (l // nx) * dy)
 
 
-def counter_record(kernel):
-    """The counter-run record of a kernel at this workload (profiles/pmc_lu_map_n1000.md):
-    HBM bytes per launch read and written (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB
-    copy) and the average duration of the kernel alone on the GPU; None if not recorded."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+def counter_record(kernel, path):
+    """The counter-run record of a kernel in the serialized pass of a map path at this
+    workload (profiles/traffic.json, from the runs of profiles/pmc_map_n1000.md): HBM bytes
+    per launch read and written (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy) and the
+    average duration of the kernel alone on the GPU; None if not recorded."""
     try:
-        return json.load(open(path))["kernels"]["msg::" + kernel]
+        rec = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
+        return rec["paths"][path]["kernels"]["msg::" + kernel]
     except (OSError, KeyError, ValueError):
         return None
 
 
-def measured_traffic(kernel):
-    rec = counter_record(kernel)
+def measured_traffic(kernel, path):
+    rec = counter_record(kernel, path)
     return None if rec is None else rec["read_bytes_per_launch"] + rec["write_bytes_per_launch"]
 
 
@@ -107,15 +109,20 @@ def host_cores():
 
 
 def host_description():
-    model = "unknown"
+    """The host CPU as /proc/cpuinfo names it (first processor): model name, and the family,
+    model and stepping numbers where the name is generic."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
+            if not line.strip():
                 break
+            key, _, val = line.partition(":")
+            info[key.strip()] = val.strip()
     except OSError:
         pass
-    return {"cpu_model": model, "cpus_online": os.cpu_count(),
+    model = info.get("model name", "unknown")
+    ids = ", ".join("%s %s" % (k, info[k]) for k in ("cpu family", "model", "stepping") if k in info)
+    return {"cpu_model": model + (" (%s)" % ids if ids else ""), "cpus_online": os.cpu_count(),
             "lapack": "Intel MKL via /opt/conda/lib/libmkl_rt.so.1 (sequential per thread)"}
 
 
@@ -505,6 +512,8 @@ def main():
         g = prof["gemm"]
         print(json.dumps({"serialized_pass_ms": 1e3 * wall, "gemm_launches": g["launches"],
                           "gemm_avg_ms": g["ms"] / max(1, g["launches"]),
+                          "gemm_alg_bytes_per_launch": g["bytes"] / max(1, g["launches"]),
+                          "gemm_flops_per_launch": g["flops"] / max(1, g["launches"]),
                           "gemm_tflops": g["flops"] / max(1e-12, g["ms"] * 1e-3) / 1e12,
                           "kernel_ms": {c: v["ms"] for c, v in prof.items() if v["launches"]},
                           "launches": {c: v["launches"] for c, v in prof.items() if v["launches"]}}))
@@ -551,16 +560,18 @@ def main():
     dtl, dre = dtracelog(out_mixed)
     dtl_re = float(dre.max())
     # the pivot-growth gate (complex64 factor; complex128 where 16 eps32 exp(max log|U_ii| -
-    # min log|U_ii|) > tol) at the 1e-8 target and at a looser tolerance
+    # min log|U_ii|) > tol, tol on |d tracelog|) at the log det target 1e-8 (tol = 1e-8 / N)
+    # and at a looser tolerance
     gated = None
     if not args.no_mixed:
         gated = {}
-        for tol in (1e-8, 1e-5):
+        for tol in (1e-8 / n, 1e-5):
             og, fg, tg, _ = timed_path(G.MIXED_GATED, tol)
             e, _ = dtracelog(og)
             gated["tol_%g" % tol] = {
                 "evals_per_s": args.steps * nk / tg, "recomputed_cells": int(fg.sum()), "cells": nk,
-                "max_abs_dtracelog": float(e.max()), "median_abs_dtracelog": float(np.median(e)),
+                "max_abs_dtracelog": float(e.max()), "max_abs_dlogdet": float(n * e.max()),
+                "median_abs_dtracelog": float(np.median(e)),
                 "p99_abs_dtracelog": float(np.percentile(e, 99)), "cells_within_tol": int((e <= tol).sum()),
                 "max_abs_dtracelog_kept_cells": float(e[fg == 0].max()) if (fg == 0).any() else 0.0}
             progress("gated mixed map, tol %g: %.1f evals/s, %d cells recomputed" % (tol, args.steps * nk / tg,
@@ -584,10 +595,11 @@ def main():
         small = small_map(G, 3, args.tune)
         progress("N=200 map done")
 
-    mixed_ok = float(dtl.max()) <= 1e-8
+    # the bound is on log det = N tracelog
+    mixed_ok = n * float(dtl.max()) <= 1e-8
     dtl_ldlt, dre_ldlt = dtracelog(outL)
-    ldlt_ok = float(dtl_ldlt.max()) <= 1e-8
-    # the fastest batched path within 1e-8 of the complex128 LU map
+    ldlt_ok = n * float(dtl_ldlt.max()) <= 1e-8
+    # the fastest batched path within 1e-8 in log det of the complex128 LU map
     cands = [(fp64_batched, "fp64_batched", prof64, PEAK_FP64_MFMA, "gemm_sub_kernel<double>", t64, G.FP64)]
     if ldlt_ok:
         cands.append((ldlt_rate, "fp64_ldlt", profL, PEAK_FP64_MFMA, "gemm_sub_kernel<double>", tL, G.FP64_LDLT))
@@ -604,12 +616,13 @@ def main():
     gs = ser[0]["gemm"] if ser else g
     achieved = gs["flops"] / (gs["ms"] * 1e-3) if gs["ms"] > 0 else 0.0
     achieved_mix = g["flops"] / (g["ms"] * 1e-3) if g["ms"] > 0 else 0.0
-    lu_counters = value_path == "fp64_batched"      # the recorded counter runs are of the LU map's GEMM
+    traffic = measured_traffic(kname, value_path)
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": measured_traffic(kname) if lu_counters else None,
-                "traffic_source": "profiles/traffic.json, from the counter runs of profiles/pmc_lu_map_n1000.md "
-                                  "(rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE per launch, scaled by a 1 GiB copy)"
-                                  if lu_counters else "no counter run of this path's launches is recorded",
+                "frac": achieved / peak, "traffic": traffic,
+                "traffic_source": "profiles/traffic.json: counter runs of the serialized pass of this path "
+                                  "(profiles/pmc_map_n1000.md; rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, per "
+                                  "launch, scaled by a 1 GiB copy)" if traffic is not None else
+                                  "no counter run of this path's launches is recorded",
                 "kernel": kname + (" (trailing updates of the LDL^T (lower tiles) and of the LU, MFMA 16x16x4)"
                                    if value_path == "fp64_ldlt" else " (LU trailing update, MFMA 16x16x4)"),
                 "path": value_path,
@@ -660,9 +673,10 @@ def main():
                       "max_abs_dtracelog_re": float(dre_ldlt.max())},
         "fp64_plain_evals_per_s": fp64_plain, "value_over_fp64_plain": value / fp64_plain if fp64_plain else None,
         "mixed_reevaluated_cells": int(flags.sum()), "mixed_reeval_rel": 1e-6,
-        "mixed_max_abs_dtracelog": float(dtl.max()), "mixed_median_abs_dtracelog": float(np.median(dtl)),
+        "mixed_max_abs_dtracelog": float(dtl.max()), "mixed_max_abs_dlogdet": float(n * dtl.max()),
+        "mixed_median_abs_dtracelog": float(np.median(dtl)),
         "mixed_p90_p99_abs_dtracelog": [float(np.percentile(dtl, 90)), float(np.percentile(dtl, 99))],
-        "mixed_cells_within_1e-8": int((dtl <= 1e-8).sum()),
+        "mixed_cells_within_1e-8_logdet": int((n * dtl <= 1e-8).sum()),
         "mixed_max_abs_dtracelog_re": dtl_re, "roofline_c64_gemm": roofline_c64,
         "mixed_gated": gated,
         "resonance_latency_ms": serving["by_kind_ms"]["resonance32"] if serving else None,

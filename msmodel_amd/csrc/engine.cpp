@@ -835,14 +835,17 @@ static void factor_ldlt(cx<double>* mats, int np, int nbatch, unsigned long long
         panel(k0);
         const int W1 = np - k0 - NB;
         if (W1 <= 0) break;
-        timed(2, 8.0 * W1 * NB * NB * nbatch, 0.0, st, [&] {
+        // algorithmic bytes: C read and written once, A and B read once
+        const double es = sizeof(cx<double>);
+        timed(2, 8.0 * W1 * NB * NB * nbatch, es * nbatch * (2.0 * W1 * NB + (double)(W1 + NB) * NB), st, [&] {
             launch_gemm_sub<double>(at(k0 + NB, k0 + NB), at(k0 + NB, k0), at(k0, k0 + NB), W1, NB, NB, np, np, np, ms,
                                     ms, ms, nbatch, st);
         });
         panel(k0 + NB);
         const int W2 = np - k0 - 2 * NB;
         if (W2 <= 0) break;
-        timed(2, 8.0 * W2 * (W2 + (double)NB) / 2.0 * (2 * NB) * nbatch, 0.0, st, [&] {
+        timed(2, 8.0 * W2 * (W2 + (double)NB) / 2.0 * (2 * NB) * nbatch,
+              es * nbatch * ((double)W2 * (W2 + NB) + 2.0 * W2 * (2 * NB)), st, [&] {
             launch_gemm_sub_lower<double>(at(k0 + 2 * NB, k0 + 2 * NB), at(k0 + 2 * NB, k0), at(k0, k0 + 2 * NB), W2,
                                           2 * NB, np, np, np, ms, ms, ms, nbatch, st);
         });
