@@ -165,7 +165,7 @@ def dropin_msmain(n, nx, ny):
         return {"evals_per_s": None, "threads": threads, "sample": "failed: %s" % ex}
 
 
-def run_msmain_conf(exe, conf_text, threads, section):
+def run_msmain_conf(exe, conf_text, threads, section, extra_env=None):
     """Runs an msmain build on a configuration; returns the realtime (s) msmain writes for
     the given section of its output."""
     tmp = tempfile.mkdtemp(prefix="msbench_")
@@ -174,6 +174,7 @@ def run_msmain_conf(exe, conf_text, threads, section):
         with open(conf, "w") as f:
             f.write(conf_text)
         env = dict(os.environ, MKL_THREADING_LAYER="GNU", MKL_NUM_THREADS="1", OMP_NUM_THREADS=str(threads))
+        env.update(extra_env or {})
         subprocess.run([exe, "-p", str(threads), conf], cwd=tmp, env=env, check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL, timeout=600)
         for dp, _, files in os.walk(os.path.join(tmp, "out")):
@@ -194,8 +195,10 @@ def run_msmain_conf(exe, conf_text, threads, section):
 def dropin_wavefunction(n, grids):
     """msmain's [wavefunction] task (the pixel loop of run_wave_function_omp calling
     eval_total_wave once per pixel from every thread): the reference msmain (MKL, host
-    threads) against msmain relinked to the library (concurrent calls coalesced on the
-    GPU); d=3, N unit-density ball, seed 1, hardsphere 0.1, plane wave k=10."""
+    threads) against msmain relinked to the library, whose calls the resident psi kernel
+    serves (and, for comparison, with MSGPU_WAVE_RESIDENT=0: concurrent calls coalesced
+    into one launch per batch); d=3, N unit-density ball, seed 1, hardsphere 0.1, plane
+    wave k=10."""
     ref = os.path.join(ROOT, "oracle", "_ref", "msmain")
     gpu = os.path.join(ROOT, "oracle", "_ref", "msmain_gpu")
     if not (os.path.exists(ref) and os.path.exists(gpu)):
@@ -209,8 +212,9 @@ def dropin_wavefunction(n, grids):
         try:
             tr = run_msmain_conf(ref, conf, threads, "wavefunction")
             tg = run_msmain_conf(gpu, conf, threads, "wavefunction")
+            tb = run_msmain_conf(gpu, conf, threads, "wavefunction", {"MSGPU_WAVE_RESIDENT": "0"})
             res["grid_%d" % g] = {"reference_s": tr, "dropin_s": tg, "dropin_over_reference": tr / tg,
-                                  "dropin_pixels_per_s": g * g / tg}
+                                  "dropin_pixels_per_s": g * g / tg, "dropin_batched_launches_s": tb}
         except Exception as ex:
             res["grid_%d" % g] = {"failed": str(ex)}
     return res

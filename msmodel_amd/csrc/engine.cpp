@@ -1888,12 +1888,172 @@ void run_wave_batch(const std::vector<WaveReq*>& B) {
     L.sync();
     for (int i = 0; i < npt; i++) B[i]->psi = hq.p[i];
 }
+
+// The resident psi kernel (launch_wave_resident): callers take consecutive request
+// indices, fill slot index % kRing of the ring in coherent pinned host memory and
+// publish it; the kernel's workgroup index % kResWG serves it and writes psi back into
+// the slot.  A caller spins on its slot's done word, relaunching the kernel when its
+// launch has ended (idle for kResIdleMs or older than kResLifeMs): each workgroup
+// resumes at its next request index, so no request is lost or served twice.  A slot is
+// reused by request index + kRing only after the caller of index has read its psi.
+// The ring and the stop word are never freed (a launch may still read them at exit);
+// the destructor sets the stop word, which ends a running launch at its next poll.
+constexpr int kRing = 4096;
+constexpr int kResWG = 64;
+constexpr double kResIdleMs = 20.0;
+constexpr double kResLifeMs = 2000.0;
+
+struct WaveResident {
+    WaveSlot* ring = nullptr;           // host addresses
+    unsigned* stop = nullptr;
+    WaveSlot* dring = nullptr;          // the same memory as the device addresses it
+    unsigned* dstop = nullptr;
+    DBuf<unsigned long long> resume, last;
+    DBuf<unsigned> closing;
+    std::unique_ptr<std::atomic<unsigned long long>[]> consumed;
+    std::atomic<unsigned long long> next{0};
+    std::atomic<bool> failed{false};
+    std::string error;
+    std::mutex mu;
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    bool launched = false;
+    long long idle_ticks = 0, life_ticks = 0;
+
+    WaveResident() {
+        int dev = 0, khz = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        if (khz <= 0) throw std::runtime_error("msgpu: the device reports no wall clock rate");
+        idle_ticks = (long long)(kResIdleMs * khz);
+        life_ticks = (long long)(kResLifeMs * khz);
+        void* h = nullptr;
+        HIPCHK(hipHostMalloc(&h, sizeof(WaveSlot) * kRing + 64, hipHostMallocCoherent | hipHostMallocMapped));
+        memset(h, 0, sizeof(WaveSlot) * kRing + 64);
+        ring = static_cast<WaveSlot*>(h);
+        stop = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(h) + sizeof(WaveSlot) * kRing);
+        void* dh = nullptr;
+        HIPCHK(hipHostGetDevicePointer(&dh, h, 0));
+        dring = static_cast<WaveSlot*>(dh);
+        dstop = reinterpret_cast<unsigned*>(reinterpret_cast<char*>(dh) + sizeof(WaveSlot) * kRing);
+        consumed.reset(new std::atomic<unsigned long long>[kRing]);
+        for (int q = 0; q < kRing; q++) consumed[q].store(0);
+        resume.alloc(kResWG);
+        last.alloc(1);
+        closing.alloc(1);
+        std::vector<unsigned long long> r0(kResWG);
+        for (int w = 0; w < kResWG; w++) r0[w] = (unsigned long long)w;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHK(hipMemcpyAsync(resume.p, r0.data(), sizeof(unsigned long long) * kResWG, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    // At process exit: the stop word ends a running launch at its next poll; wait for it
+    // briefly so that the process does not end with the launch still reading the ring.
+    void shutdown() {
+        __atomic_store_n(stop, 1u, __ATOMIC_RELEASE);
+        std::lock_guard<std::mutex> g(mu);
+        for (int t = 0; launched && t < 1000 && hipEventQuery(ev) == hipErrorNotReady; t++)
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+    }
+
+    // Launches the kernel unless a launch is running; false (and the error kept) if it
+    // cannot, after which every waiting caller gives up.
+    bool ensure_running() {
+        std::lock_guard<std::mutex> g(mu);
+        if (failed.load()) return false;
+        try {
+            if (launched) {
+                const hipError_t q = hipEventQuery(ev);
+                if (q == hipErrorNotReady) return true;
+                HIPCHK(q);
+            }
+            HIPCHK(hipMemsetAsync(closing.p, 0, sizeof(unsigned), st));
+            HIPCHK(hipMemsetAsync(last.p, 0, sizeof(unsigned long long), st));
+            launch_wave_resident(dring, kRing, dstop, resume.p, closing.p, last.p, idle_ticks, life_ticks, kResWG, st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ev, st));
+            launched = true;
+            return true;
+        } catch (const std::exception& ex) {
+            error = ex.what();
+            failed.store(true);
+            return false;
+        }
+    }
+
+    cx<double> request(const WaveCtx& c, const double* x) {
+        const unsigned long long i = next.fetch_add(1);
+        const int s = (int)(i % kRing);
+        if (i >= (unsigned long long)kRing)
+            while (consumed[s].load(std::memory_order_acquire) != i - kRing + 1) std::this_thread::yield();
+        WaveSlot& sl = ring[s];
+        sl.pos = c.dpos.p;
+        sl.a = c.da.p;
+        for (int q = 0; q < 4; q++) sl.x[q] = q < c.d ? x[q] : 0.0;
+        sl.k = c.k;
+        sl.n = c.n;
+        sl.d = c.d;
+        sl.source = c.source;
+        __atomic_store_n(&sl.seq, i + 1, __ATOMIC_RELEASE);
+        bool ok = ensure_running();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (unsigned spins = 1; ok && __atomic_load_n(&sl.done, __ATOMIC_ACQUIRE) != i + 1; spins++) {
+            if ((spins & 4095) != 0) {
+                __builtin_ia32_pause();
+                continue;
+            }
+            ok = ensure_running();      // the launch may have ended before serving i
+            if (ok && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(10)) {
+                // a request left unserved would hold up every later one of its workgroup:
+                // stop the kernel and fail every call from here on
+                std::lock_guard<std::mutex> g(mu);
+                error = "request not served within 10 s";
+                failed.store(true);
+                __atomic_store_n(stop, 1u, __ATOMIC_RELEASE);
+                ok = false;
+            }
+        }
+        const cx<double> v = sl.psi;
+        consumed[s].store(i + 1, std::memory_order_release);
+        if (!ok) throw std::runtime_error("resident psi kernel: " + error);
+        return v;
+    }
+};
+
+static bool wave_resident_enabled() {
+    static const bool on = [] {
+        const char* e = getenv("MSGPU_WAVE_RESIDENT");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+static WaveResident* g_wave_resident = nullptr;
+
+static WaveResident& wave_resident() {
+    // never destroyed (a launch may outlive static destruction); stopped at exit
+    static WaveResident* r = [] {
+        WaveResident* w = new WaveResident();
+        g_wave_resident = w;
+        atexit([] { g_wave_resident->shutdown(); });
+        return w;
+    }();
+    return *r;
+}
 }  // namespace
 
 int wave_point_coalesced(const MediumView& mv, cx<double> k, int source, const cx<double>* a, const double* x,
                          cx<double>* psi) {
     if (mv.d < 1 || mv.d > 4) { set_error("msgpu: the device path supports d = 1..4"); return -3; }
     return guarded([&]() -> int {
+        // The calling thread's last content, kept by the thread itself: the common case (the
+        // same content as its previous call) takes no lock and touches no shared counter.
+        thread_local std::shared_ptr<WaveCtx> t_ctx;
+        if (t_ctx && t_ctx->same(mv, k, source, a) && wave_resident_enabled()) {
+            *psi = wave_resident().request(*t_ctx, x);
+            return 0;
+        }
         std::shared_ptr<WaveCtx> ctx;
         {
             std::lock_guard<std::mutex> g(g_wv_mu);
@@ -1917,6 +2077,11 @@ int wave_point_coalesced(const MediumView& mv, cx<double> k, int source, const c
             std::lock_guard<std::mutex> g(g_wv_mu);
             if (g_wv_ctx && g_wv_ctx->same(mv, k, source, a)) ctx = g_wv_ctx;   // another thread was first
             else g_wv_ctx = ctx = nc;
+        }
+        if (wave_resident_enabled()) {
+            t_ctx = ctx;
+            *psi = wave_resident().request(*ctx, x);
+            return 0;
         }
         WaveReq req;
         req.ctx = ctx;

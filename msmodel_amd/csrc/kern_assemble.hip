@@ -471,6 +471,127 @@ void launch_wave_points(const double* pos, int n, int d, const cx<double>* a, cx
         hipLaunchKernelGGL(wave_points_kernel, dim3(npt), dim3(256), 0, st, pos, n, d, a, k, source, pts, psi);
 }
 
+// ------------------------------------------------------------------ resident psi kernel
+// Serves one-point psi requests from a ring of slots in coherent pinned host memory, so
+// that a drop-in eval_total_wave call costs a round trip over the host link instead of
+// a kernel launch and a stream synchronisation.  Workgroup w serves the request indices
+// i = w, w + G, w + 2G, ... (G workgroups) in order, starting from resume[w]: thread 0
+// polls slot i % nring until the host publishes request i (seq = i + 1), the workgroup
+// sums psi at its point over the scatterers (one workgroup per point, as
+// wave_points_kernel), and thread 0 writes psi, then done = i + 1 (release, system scope).
+// Every workgroup leaves the loop, recording its next index in resume[w], when
+//  - the host sets *stop, or
+//  - no request has been served for idle_ticks (of the constant wall clock) since the
+//    later of the last one served by any workgroup and this workgroup's start, or
+//  - the launch has run for life_ticks;
+// the first workgroup to find the launch idle or too old sets *closing, which ends the
+// others at their next poll, so the launch drains as a whole and the host relaunches it
+// for the requests that arrive after.  *closing and *last are reset by the host before
+// each launch.
+__device__ __forceinline__ unsigned long long ld_sys(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ __launch_bounds__(256) void wave_resident_kernel(WaveSlot* ring, int nring, const unsigned* stop,
+                                                            unsigned long long* resume, unsigned* closing,
+                                                            unsigned long long* last, long long idle_ticks,
+                                                            long long life_ticks) {
+    __shared__ double sre[256], sim[256];
+    __shared__ const double* s_pos;
+    __shared__ const cx<double>* s_a;
+    __shared__ double s_x[4];
+    __shared__ cx<double> s_k;
+    __shared__ int s_n, s_d, s_src, s_cmd;
+    const int w = blockIdx.x, G = gridDim.x, tid = threadIdx.x;
+    unsigned long long i = resume[w];
+    const long long t0 = wall_clock64();
+    for (;;) {
+        if (tid == 0) {
+            int cmd = 0;                                  // 1: serve request i, 2: leave
+            WaveSlot* sl = ring + (i % (unsigned long long)nring);
+            while (cmd == 0) {
+                if (__hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) ||
+                    __hip_atomic_load(closing, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    cmd = 2;
+                } else if (ld_sys(&sl->seq) == i + 1) {
+                    cmd = 1;
+                } else {
+                    const long long now = wall_clock64();
+                    const long long ref = max(t0, (long long)__hip_atomic_load(last, __ATOMIC_RELAXED,
+                                                                                 __HIP_MEMORY_SCOPE_AGENT));
+                    if (now - ref > idle_ticks || now - t0 > life_ticks) {
+                        __hip_atomic_store(closing, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        cmd = 2;
+                    } else {
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                }
+            }
+            if (cmd == 1) {
+                s_pos = sl->pos;
+                s_a = sl->a;
+                for (int c = 0; c < 4; c++) s_x[c] = sl->x[c];
+                s_k = sl->k;
+                s_n = sl->n;
+                s_d = sl->d;
+                s_src = sl->source;
+            }
+            s_cmd = cmd;
+        }
+        __syncthreads();
+        if (s_cmd == 2) break;
+        const int n = s_n, d = s_d;
+        const cx<double> k = s_k;
+        const double* __restrict__ pos = s_pos;
+        const cx<double>* __restrict__ a = s_a;
+        double pt[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) pt[c] = c < d ? s_x[c] : 0.0;
+        cx<double> acc{0.0, 0.0};
+        for (int q = tid; q < n; q += 256) {
+            double s2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const double e = (c < d ? pos[(size_t)q * d + c] : 0.0) - pt[c];
+                s2 += e * e;
+            }
+            acc = acc + a[q] * (d == 3 ? -neg_green3_s<double, false>(k, s2) : green<double>(d, k, sqrt(s2)));
+        }
+        sre[tid] = acc.x;
+        sim[tid] = acc.y;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (tid < h) { sre[tid] += sre[tid + h]; sim[tid] += sim[tid + h]; }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            cx<double> inc;
+            if (s_src == 0) {
+                inc = cexp(cx<double>{-k.y * pt[0], k.x * pt[0]});
+            } else {
+                double r2 = 0.0;
+                for (int c = 0; c < d; c++) r2 += pt[c] * pt[c];
+                inc = green<double>(d, k, sqrt(r2));
+            }
+            WaveSlot* sl = ring + (i % (unsigned long long)nring);
+            sl->psi = cx<double>{sre[0], sim[0]} + inc;
+            __hip_atomic_store(&sl->done, i + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_fetch_max(last, (unsigned long long)wall_clock64(), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        i += (unsigned long long)G;
+        __syncthreads();
+    }
+    if (tid == 0) resume[w] = i;
+}
+
+void launch_wave_resident(WaveSlot* ring, int nring, const unsigned* stop, unsigned long long* resume,
+                          unsigned* closing, unsigned long long* last, long long idle_ticks, long long life_ticks,
+                          int nwg, hipStream_t st) {
+    hipLaunchKernelGGL(wave_resident_kernel, dim3(nwg), dim3(256), 0, st, ring, nring, stop, resume, closing, last,
+                       idle_ticks, life_ticks);
+}
+
 void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int nx, int ny,
                  int row0, double xmin, double dx, double ymax, double dy, const double* pts, cx<double>* psi,
                  hipStream_t st) {

@@ -45,6 +45,27 @@ void launch_incident(const double* pos, int n, int d, int np, cx<double> k, int 
 // psi at npt points (row-major [npt][d]), one workgroup per point (small batches)
 void launch_wave_points(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int npt,
                         const double* pts, cx<double>* psi, hipStream_t st);
+// One request of the resident psi kernel (a slot of its ring in coherent pinned host
+// memory): the host fills the request fields and then publishes seq = index + 1; the
+// device writes psi and then done = index + 1.
+struct WaveSlot {
+    const double* pos;          // device copies of the positions [n][d] and amplitudes [n]
+    const cx<double>* a;
+    double x[4];                // the point
+    cx<double> k;
+    int n, d, source, pad_;
+    unsigned long long seq;
+    cx<double> psi;
+    unsigned long long done;
+    unsigned long long pad2_[2];
+};
+static_assert(sizeof(WaveSlot) == 128, "one slot per 128 bytes");
+// Workgroup w serves requests w, w + nwg, ... from resume[w] until *stop, an idle
+// launch (idle_ticks of the wall clock without a request served) or life_ticks; then
+// resume[w] holds its next request.  *closing and *last must be 0 at launch.
+void launch_wave_resident(WaveSlot* ring, int nring, const unsigned* stop, unsigned long long* resume,
+                          unsigned* closing, unsigned long long* last, long long idle_ticks, long long life_ticks,
+                          int nwg, hipStream_t st);
 // pts == nullptr: rows row0 .. row0+ny-1 of a grid nx wide (psi holds those rows);
 // otherwise nx points given row-major [nx][d] (ny and row0 ignored)
 void launch_wave(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int nx, int ny,
