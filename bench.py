@@ -382,11 +382,20 @@ def small_map(G, steps, tune):
             res, _ = G.tracelog_batch(med, ks, G.FP64)
         out["graph" if graph else "eager"] = steps * ks.size / (time.perf_counter() - t0)
         out["tracelog_%d" % graph] = res
-    same = bool(np.array_equal(out.pop("tracelog_0"), out.pop("tracelog_1")))
+    lu = out.pop("tracelog_0")
+    same = bool(np.array_equal(lu, out.pop("tracelog_1")))
     set_entry(G, 256, 0, tuned)
+    # the pivot-free LDL^T path on the same map (LU where its growth is large)
+    G.tracelog_batch(med, ks, G.FP64_LDLT)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res, flags = G.tracelog_batch(med, ks, G.FP64_LDLT)
+    ldlt_rate = steps * ks.size / (time.perf_counter() - t0)
+    dlog = np.hypot(200 * (res.real - lu.real), (200 * (res.imag - lu.imag) + np.pi) % (2 * np.pi) - np.pi)
     return {"workload": "tracelog map d=3 N=200 ball R=1 seed 1 hardsphere 0.1, 128x64 k over Re[1,20] x Im[-1,0]",
             "evals_per_s_eager": out["eager"], "evals_per_s_graph": out["graph"], "graph_equals_eager": same,
-            "launch_shapes": tuned}
+            "evals_per_s_ldlt": ldlt_rate, "ldlt_lu_cells": int(flags.sum()),
+            "ldlt_max_abs_dlogdet": float(dlog.max()), "launch_shapes": tuned}
 
 
 def serialized_pass(G, med, ks, np_, mode):

@@ -312,7 +312,7 @@ static int max_batch(int np, size_t elem, int mats_per_item) {
     size_t per = (size_t)np * np * elem * mats_per_item;
     size_t b = batch_budget() / per;
     if (b < 1) b = 1;
-    if (b > 512) b = 512;
+    if (b > 2048) b = 2048;
     return (int)b;
 }
 
@@ -1105,7 +1105,8 @@ int autotune(int np, int prec, int nitems, double* report) {
         // matrices per launch (several per CU)
         const Tune base = best;
         for (int pv = 0; pv < kPanelVariants; pv++)
-            for (int c : {128, 256, 512}) {
+            for (int c : {128, 256, 512, 1024, 2048}) {
+                if (c > 512 && (c > nitems / 2 || c > max_batch(np, prec == 0 ? 16 : 8, 1))) continue;
                 Tune t = base;
                 t.panel = pv;
                 t.chunk = c;

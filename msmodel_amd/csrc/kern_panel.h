@@ -508,7 +508,11 @@ static void panel_dispatch(cx<T>* A, int np, long long mstride, int* ipiv, int* 
         variant = 0;
     }
     if (variant == 3) {          // 256 threads: leaves room on the CU for GEMM waves of another stream
-        if (np <= 1024) panel_go<T, RM, 8, 4, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        // (as many register rows as the padded size needs: rows beyond np would still cost
+        // issue cycles in the pivot search and the rank-1 updates)
+        if (np <= 256) panel_go<T, RM, 8, 1, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else if (np <= 512) panel_go<T, RM, 8, 2, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
+        else if (np <= 1024) panel_go<T, RM, 8, 4, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
         else if (np <= 2048) panel_go<T, RM, 4, 8, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
         else if (np <= 4096) panel_go<T, RM, 2, 16, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);
         else panel_go<T, RM, 1, 32, 256, LL>(A, np, mstride, ipiv, info, pmap, k0, nb, nbatch, st, xa, xb);

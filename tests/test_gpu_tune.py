@@ -107,13 +107,14 @@ def test_graph_replays_match_eager(G, tmp_path, prec_mode):
 
 
 @pytest.mark.parametrize("prec_mode", ["FP64", "FP32"])
-@pytest.mark.parametrize("n,np_,variants", [(1000, 1024, (0, 1, 2, 3, 4, 5, 6, 7)), (1100, 1152, (0, 1, 2, 3, 5, 6)),
+@pytest.mark.parametrize("n,np_,variants", [(200, 256, (0, 1, 2, 3, 4, 5, 6, 7)), (500, 512, (0, 1, 2, 3, 4, 5, 6, 7)),
+                                             (1000, 1024, (0, 1, 2, 3, 4, 5, 6, 7)), (1100, 1152, (0, 1, 2, 3, 5, 6)),
                                              (2000, 2048, (0, 3, 5, 6))])
 def test_panel_shapes_beyond_one_register_row(G, tmp_path, prec_mode, n, np_, variants):
-    """Panel shapes whose threads hold several rows of the panel (Np above the thread
-    count: the rank-S update stages one register row at a time), right-looking (0-4) and
-    left-looking (5-7): every shape gives the same trace-log, row-major with 128-column
-    blocks, and one cell matches the oracle."""
+    """Panel shapes whose threads hold one or several rows of the panel (Np above the
+    thread count: the rank-S update stages one register row at a time), right-looking (0-4)
+    and left-looking (5-7), 256 to 1024 threads: every shape gives the same trace-log,
+    row-major with 128-column blocks, and one cell matches the oracle."""
     mode = getattr(G, prec_mode)
     prec = 0 if mode == G.FP64 else 1
     pos = O.fill_medium(3, n, 12)
