@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
             const int i = 4 * m + qq;
             xv[m] = (live && i < nb) ? M[at<RM>(k0 + i, col, np)] : cx<T>{T(0), T(0)};
         }
-        lower_unit_solve<T>(xv, PackedL<T>{sL}, nb, qq, lane);
+        lower_unit_solve_unrolled<T>(xv, PackedL<T>{sL}, nb, qq);
         if (live) {
 #pragma unroll
             for (int m = 0; m < 16; m++) {

@@ -3,6 +3,8 @@
 // the factorisations, DPP/readlane arg-max reductions of the pivot search, and the
 // in-register triangular solves (four lanes per right-hand side).
 #pragma once
+#include <utility>
+
 #include "kernels.h"
 
 namespace msg {
@@ -172,6 +174,43 @@ __device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], LA sL, int qq, int 
         for (int m = 15; m > 0; m--) xv[m] = xv[m - 1];
         xv[0] = t;
     }
+}
+
+// The unit-diagonal forward solve with the 64 steps unrolled: same lanes and registers,
+// and each row receives its updates in the same order, so the results are the same bits.
+// Every register index, step i and row offset is then a compile-time constant, and an
+// element sL(4m+qq, i) of the packed triangle is read at a per-lane base of register m
+// plus a constant offset, with no address arithmetic per access (the rolled loop above,
+// whose register file rotates, issues several times the multiply-adds in instructions,
+// mostly for those addresses).  Called by whole waves.  Kernels that run several solves
+// over the same triangle keep the rolled form (unrolled, they spill); so do the solves
+// with a stored diagonal, whose unrolled form the compiler neither unrolls nor (forced)
+// keeps in registers.
+// One step i of it; the steps are expanded from a parameter pack, so each is unrolled
+// by construction.
+template <class T, int I, class LA>
+__device__ __forceinline__ void lower_step(cx<T> (&xv)[16], LA sL, int nb, int qq) {
+    constexpr int mi = I >> 2, iq = I & 3;
+    const cx<T> xi{quad_bcast(xv[mi].x, iq), quad_bcast(xv[mi].y, iq)};
+#pragma unroll
+    for (int m = mi; m < 16; m++) {
+        const int row = 4 * m + qq;
+        if (row > I && row < nb) xv[m] = xv[m] - sL(row, I) * xi;
+    }
+    // every 4 steps: later steps' loads may not move above this point (unbounded, the
+    // scheduler keeps hundreds of them in flight and spills)
+    if (iq == 3) __builtin_amdgcn_sched_barrier(0);
+}
+
+template <class T, class LA, int... Is>
+__device__ __forceinline__ void lower_steps(cx<T> (&xv)[16], LA sL, int nb, int qq, std::integer_sequence<int, Is...>) {
+    (lower_step<T, Is>(xv, sL, nb, qq), ...);
+}
+
+// Unit-diagonal forward substitution, unrolled (see above).
+template <class T, class LA>
+__device__ __forceinline__ void lower_unit_solve_unrolled(cx<T> (&xv)[16], LA sL, int nb, int qq) {
+    lower_steps<T>(xv, sL, nb, qq, std::make_integer_sequence<int, 64>{});
 }
 
 }  // namespace msg
