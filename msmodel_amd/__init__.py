@@ -163,7 +163,7 @@ def init():
     return _check(lib.msgpu_init(), "msgpu_init")
 
 
-PROFILE_CATEGORIES = ("panel", "swap_trsm", "gemm", "assembly", "tracelog", "psi")
+PROFILE_CATEGORIES = ("panel", "swap_trsm", "gemm", "assembly", "tracelog", "psi", "persistent")
 
 
 def profile_begin():

@@ -852,11 +852,76 @@ static void factor_ldlt(cx<double>* mats, int np, int nbatch, unsigned long long
     }
 }
 
+// The LDL^T trace-log of every item in one persistent launch (kern_persist.hip): the
+// items in order of decreasing size (the largest start first), as many workgroups as stay
+// resident on the GPU (within the workspace budget, one Np x Np matrix each).  Used for
+// batches whose largest padded size is at most MSGPU_LDLT_PERSIST_NP (default 0: never).
+// Its results equal the per-phase launches' bit for bit, but it measured slower on the
+// MI355X (d = 3 maps: 0.61x at N = 200, 0.73x at N = 500, 0.80x at N = 1000): one
+// workgroup carries an item's latency-bound phases and its GEMM tiles one after the
+// other, where the per-phase launches spread each phase over all CUs.
+static int ldlt_persist_np() {
+    static const int v = [] {
+        const char* e = getenv("MSGPU_LDLT_PERSIST_NP");
+        return e ? atoi(e) : 0;
+    }();
+    return v;
+}
+
+static void tracelog_ldlt_persist(const std::vector<MediumView>& meds, int nk, const int* midx,
+                                  const cx<double>* ks, cx<double>* out, double* stats) {
+    Lease L;
+    std::vector<int> order(nk);
+    for (int i = 0; i < nk; i++) order[i] = i;
+    auto np_of = [&](int i) { return meds[midx ? midx[i] : 0].np; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return np_of(a) > np_of(b); });
+    DeviceItems D(meds, nk, midx, ks, order, L.st);
+    const int npmax = np_of(order[0]);
+    const bool d3 = D.dim(0, nk) == 3;
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long ms = (long long)npmax * npmax;
+    const int per_cu = std::max(1, persist_ldlt_blocks_per_cu(d3));
+    const int nwg = std::max(1, std::min({nk, ncu * per_cu,
+                                          (int)std::min<size_t>(1 << 20, batch_budget() / (ms * sizeof(cx<double>)))}));
+    DBuf<cx<double>> ws((size_t)nwg * ms), dout(nk);
+    DBuf<unsigned long long> dgrow(nk);
+    DBuf<int> counter(1);
+    HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), L.st));
+    double flops = 0.0;        // algorithmic: 4 N^3 / 3 per complex LDL^T
+    for (int i = 0; i < nk; i++) {
+        const double n = meds[midx ? midx[i] : 0].n;
+        flops += 4.0 * n * n * n / 3.0;
+    }
+    timed(6, flops, 0.0, L.st,
+          [&] { launch_ldlt_persist(D.items.p, nk, d3, ws.p, ms, nwg, counter.p, dout.p, dgrow.p, L.st); });
+    HIPCHK(hipGetLastError());
+    std::vector<cx<double>> hout(nk);
+    std::vector<unsigned long long> hgrow(nk);
+    HIPCHK(hipMemcpyAsync(hout.data(), dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, L.st));
+    HIPCHK(hipMemcpyAsync(hgrow.data(), dgrow.p, sizeof(unsigned long long) * nk, hipMemcpyDeviceToHost, L.st));
+    L.sync();
+    for (int q = 0; q < nk; q++) {
+        out[order[q]] = hout[q];
+        if (stats) {
+            memcpy(&stats[2 * order[q]], &hgrow[q], sizeof(double));
+            stats[2 * order[q] + 1] = 0.0;
+        }
+    }
+}
+
 // ldlt (complex128 only): the pivot-free LDL^T instead of the LU; stats[2 l] receives the
 // growth max |L| of cell l (stats[2 l + 1] = 0).
 template <class T>
 static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
                           cx<double>* out, double* stats, bool ldlt = false) {
+    if constexpr (std::is_same<T, double>::value) {
+        if (ldlt && max_np(meds, nk, midx) <= ldlt_persist_np()) {
+            tracelog_ldlt_persist(meds, nk, midx, ks, out, stats);
+            return;
+        }
+    }
     Lanes S(streams_for(max_np(meds, nk, midx), prec_of<T>()));
     const Plan P = make_plan(meds, nk, midx, sizeof(cx<T>), 1, S.ns);
     DeviceItems D(meds, nk, midx, ks, P.order, S[0]);

@@ -7,6 +7,7 @@
 // zeros elsewhere), which leaves the determinant, the pivot sequence of the
 // first N columns and the solution of the first N unknowns unchanged.
 #include "kernels.h"
+#include "assemble_dev.h"
 
 namespace msg {
 
@@ -48,37 +49,8 @@ void launch_item_diag(MatItem* items, int cnt, hipStream_t st) {
     if (cnt > 0) hipLaunchKernelGGL(item_diag_kernel, dim3((cnt + 63) / 64), dim3(64), 0, st, items, cnt);
 }
 
-// ------------------------------------------------------------------ d = 3 Green function
-__device__ __forceinline__ double rsqrt_t(double x) { return rsqrt(x); }
-__device__ __forceinline__ float rsqrt_t(float x) { return rsqrtf(x); }
-
-// -G_3(k, r) = e^{ikr} / (4 pi r) from s = r^2, with 1/r from one reciprocal square root
-// and r = s * (1/r), in place of a square root and a division; REALK (Im k = 0) leaves out
-// the factor exp(-Im k r) = 1.  At s = 0 it returns what the general form does there.
-template <class T, bool REALK>
-__device__ __forceinline__ cx<T> neg_green3_s(cx<T> k, T s) {
-    const T ri = rsqrt_t(s);
-    const T r = s > T(0) ? s * ri : T(0);
-    T sn, cs;
-    sincos_t(k.x * r, &sn, &cs);
-    T w = ri * T(1.0 / kFourPi);
-    if (!REALK) w *= exp(-k.y * r);
-    return {cs * w, sn * w};
-}
-
 // ------------------------------------------------------------------ assembly
-// One workgroup per pair of 32x32 tiles (I <= J) of one matrix: it evaluates the
-// Green function once per unordered pair and writes both (I,J) and, through an
-// LDS transpose, (J,I), so M = M^T exactly (ms_matrix_util.c:36-41 mirrors the
-// upper triangle the same way).
-//
-// D3: every matrix of the launch is d = 3 and kind 0.  The general kernel carries the
-// Bessel-function code of d = 2 and 4 and needs 208 VGPRs (two waves per SIMD), which
-// leaves the d = 3 stores little to overlap with; the specialised one has only the d = 3
-// Green function.
-constexpr int AT = 32;                 // tile edge (256 threads: AT / (256 / AT) elements each)
-constexpr int ARW = 256 / AT;           // thread rows of a tile
-
+// One workgroup per pair of 32x32 tiles (I <= J) of one matrix (assemble_pair).
 template <class T, bool D3>
 __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict__ pos, int n, int d, int np,
                                                        const cx<double>* __restrict__ ks,
@@ -99,69 +71,13 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
         kd = ks[b];
         dg = diag[b];
     }
-    const int nt = np / AT;
-    // decode the upper-triangular tile pair index
-    int t = blockIdx.x, I = 0;
-    while (t >= nt - I) { t -= nt - I; I++; }
-    const int J = I + t;
+    int I, J;
+    tile_pair(blockIdx.x, np / AT, I, J);
     __shared__ double sx[2][AT][4];
     __shared__ cx<T> tile[AT][AT + 1];
-    const int tid = threadIdx.x;
-    if (tid < 2 * AT) {
-        int side = tid / AT, a = tid % AT;
-        int atom = (side == 0 ? I : J) * AT + a;
-        for (int c = 0; c < 4; c++) sx[side][a][c] = (atom < n && c < d) ? pos[(size_t)atom * d + c] : 0.0;
-    }
-    __syncthreads();
-    const cx<T> k{(T)kd.x, (T)kd.y};
     cx<double> iz{1.0, 0.0};
     if (kind == 2) iz = izero[b];
-    cx<T>* M = out + (size_t)b * mstride;
-    const int ii = tid % AT, j0 = tid / AT;
-#pragma unroll
-    for (int q = 0; q < AT / ARW; q++) {
-        const int jj = j0 + ARW * q;
-        const int gi = I * AT + ii, gj = J * AT + jj;
-        cx<T> v;
-        if (gi >= n || gj >= n) {
-            v = cx<T>{gi == gj ? T(1) : T(0), T(0)};
-        } else if (gi == gj) {
-            v = cx<T>{(T)dg.x, (T)dg.y};
-        } else if (D3 || (d == 3 && kind == 0)) {
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                double e = sx[0][ii][c] - sx[1][jj][c];
-                s += e * e;
-            }
-            v = neg_green3_s<T, false>(k, (T)s);
-        } else if constexpr (!D3) {
-            double s = 0.0;
-            for (int c = 0; c < d; c++) {
-                double e = sx[0][ii][c] - sx[1][jj][c];
-                s += e * e;
-            }
-            T r = (T)sqrt(s);
-            if (kind == 1) {
-                v = neg_dk_green<T>(d, k, r);
-            } else if (kind == 2) {
-                const cx<T> ng = neg_green<T>(d, k, r);
-                const cx<double> g = cx<double>{(double)ng.x, (double)ng.y} / iz;
-                v = cx<T>{(T)g.x, (T)g.y};
-            } else {
-                v = neg_green<T>(d, k, r);
-            }
-        }
-        M[(size_t)gi + (size_t)gj * np] = v;
-        tile[jj][ii] = v;
-    }
-    if (I == J) return;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < AT / ARW; q++) {
-        const int bb = j0 + ARW * q;      // column inside tile (J, I) is I*AT + bb
-        M[(size_t)(J * AT + ii) + (size_t)(I * AT + bb) * np] = tile[ii][bb];
-    }
+    assemble_pair<T, D3>(pos, n, d, np, kd, dg, iz, kind, out + (size_t)b * mstride, I, J, sx, tile);
 }
 
 template <class T>

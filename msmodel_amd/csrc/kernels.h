@@ -45,6 +45,14 @@ void launch_incident(const double* pos, int n, int d, int np, cx<double> k, int 
 // psi at npt points (row-major [npt][d]), one workgroup per point (small batches)
 void launch_wave_points(const double* pos, int n, int d, const cx<double>* a, cx<double> k, int source, int npt,
                         const double* pts, cx<double>* psi, hipStream_t st);
+// Persistent trace-log by the pivot-free LDL^T (kern_persist.hip): nwg workgroups take
+// items[0..nitems) from *counter (0 at launch) and each runs an item's assembly,
+// factorisation and reduction in its own workspace matrix (ws + w * wstride, at least
+// Np^2 elements for the largest item); out[i] = sum log D_ii / N, growth[i] = the bits of
+// its largest |L|.  d3: every item is d = 3.
+int persist_ldlt_blocks_per_cu(bool d3);
+void launch_ldlt_persist(const MatItem* items, int nitems, bool d3, cx<double>* ws, long long wstride, int nwg,
+                         int* counter, cx<double>* out, unsigned long long* growth, hipStream_t st);
 // One request of the resident psi kernel (a slot of its ring in coherent pinned host
 // memory): the host fills the request fields and then publishes seq = index + 1; the
 // device writes psi and then done = index + 1.

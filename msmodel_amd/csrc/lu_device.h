@@ -96,6 +96,47 @@ __device__ __forceinline__ void key_wave(T& v, int& i) {
     i = bi;
 }
 
+// ------------------------------------------------------------------ trace-log of a factor
+// *out = (sum_i log U_ii + i pi [odd interchanges]) / n  (ms_matrix_util.c:185-202) for
+// one factor M (ld np) by a 256-thread workgroup, accumulated in FP64; stats (may be
+// null) = {max_i log|U_ii|, min_i log|U_ii|}; ipiv null: no interchanges (LDL^T: the
+// diagonal is D).  s*: 256 entries each of shared memory.
+template <class T>
+__device__ __forceinline__ void tracelog_block(const cx<T>* __restrict__ M, int n, int np,
+                                               const int* __restrict__ ipiv, double* sre, double* sim,
+                                               double* smax, double* smin, int* sodd, cx<double>* out,
+                                               double* stats) {
+    const int tid = threadIdx.x;
+    double re = 0.0, im = 0.0, mx = -INFINITY, mn = INFINITY;
+    int odd = 0;
+    for (int i = tid; i < n; i += 256) {
+        const cx<T> u = M[(size_t)i + (size_t)i * np];
+        const double la = log(hypot((double)u.x, (double)u.y));
+        re += la;
+        im += atan2((double)u.y, (double)u.x);
+        mx = fmax(mx, la);
+        mn = fmin(mn, la);
+        if (ipiv) odd ^= (ipiv[i] != i + 1);
+    }
+    sre[tid] = re; sim[tid] = im; smax[tid] = mx; smin[tid] = mn; sodd[tid] = odd;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            sre[tid] += sre[tid + s];
+            sim[tid] += sim[tid + s];
+            smax[tid] = fmax(smax[tid], smax[tid + s]);
+            smin[tid] = fmin(smin[tid], smin[tid + s]);
+            sodd[tid] ^= sodd[tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double imt = sim[0] + (sodd[0] ? kPi : 0.0);
+        *out = cx<double>{sre[0] / n, imt / n};
+        if (stats) { stats[0] = smax[0]; stats[1] = smin[0]; }
+    }
+}
+
 // ------------------------------------------------------------------ in-register triangular solves
 // Value of lane q of each quad (DPP quad_perm [q,q,q,q]); q is a constant after unrolling.
 template <class T>

@@ -91,3 +91,46 @@ def test_growth_rejection_falls_back(G):
     ldl, flags = G.tracelog_batch(med, ks, G.FP64_LDLT)
     assert flags.tolist() == [1, 1, 1]
     assert np.array_equal(ldl, lu)
+
+
+def _persistent_case(G):
+    """A d=3 map at N = 300 (Np = 320, five 64-column panels) and an uneven multi-medium
+    batch (d = 2 and 3, N from 90 to 450, all padded sizes below 512)."""
+    med = G.Medium(3, 300, G.HARDSPHERE, (0.05,)).fill(5)
+    ks = kmap(24, 12, 4.0, 24.0, -1.2, 0.0)
+    meds = [G.Medium(3 if i % 2 else 2, n, G.HARDSPHERE, (0.1,)).fill(100 + i)
+            for i, n in enumerate((150, 230, 310, 450, 90))]
+    rng = np.random.default_rng(7)
+    midx = rng.integers(0, len(meds), 160).astype(np.int32)
+    mks = rng.uniform(2, 20, 160) - 1j * rng.uniform(0, 0.6, 160)
+    a, fa = G.tracelog_batch(med, ks, G.FP64_LDLT)
+    b, fb = G.tracelog_multi(meds, midx, mks, G.FP64_LDLT)
+    return a, fa, b, fb
+
+
+def test_persistent_kernel_equals_per_phase_launches(G, tmp_path):
+    """MSGPU_LDLT_PERSIST_NP: one persistent launch, each workgroup taking items from a
+    counter and running assembly, LDL^T and reduction alone, gives the per-phase launches'
+    results bit for bit (in a process of its own: the variable is read once)."""
+    import os
+    import subprocess
+    import sys
+    from conftest import ROOT
+    code = (
+        "import sys, numpy as np\n"
+        "sys.path.insert(0, %r); sys.path.insert(0, %r); sys.path.insert(0, %r)\n"
+        "import test_gpu_ldlt as T, msmodel_amd as G\n"
+        "G.init()\n"
+        "a, fa, b, fb = T._persistent_case(G)\n"
+        "np.savez(%r, a=a, fa=fa, b=b, fb=fb)\n"
+        % (ROOT, os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle"), str(tmp_path / "p.npz")))
+    env = dict(os.environ, MSGPU_LDLT_PERSIST_NP="512")
+    print("running a python process with MSGPU_LDLT_PERSIST_NP=512", flush=True)
+    subprocess.run([sys.executable, "-c", code], env=env, check=True, timeout=240)
+    p = np.load(str(tmp_path / "p.npz"))
+    a, fa, b, fb = _persistent_case(G)
+    assert np.array_equal(p["a"], a) and np.array_equal(p["fa"], fa)
+    assert np.array_equal(p["b"], b) and np.array_equal(p["fb"], fb)
+    lu, _ = G.tracelog_batch(G.Medium(3, 300, G.HARDSPHERE, (0.05,)).fill(5), kmap(24, 12, 4.0, 24.0, -1.2, 0.0),
+                             G.FP64)
+    check(a, fa, lu, 300)
