@@ -74,7 +74,14 @@ std::vector<MediumView> views_of(int nmed, const msgpu_medium* const* meds) {
 // the largest distance from the centroid) over the diagonal; it follows the measured max
 // |L| within a factor of about 2 over d=3 k-maps.  With max |L| <= 1e3 the LDL^T log det
 // agreed with the pivoted LU to 2e-11 on the N = 1000 map (64 sampled cells).
-constexpr double kLdltPredict = 300.0, kLdltAccept = 1000.0;
+// MSGPU_LDLT_PREDICT and MSGPU_LDLT_ACCEPT override the two bounds.
+static double ldlt_bound(const char* var, double dflt) {
+    const char* e = getenv(var);
+    const double v = e ? atof(e) : 0.0;
+    return v > 0.0 ? v : dflt;
+}
+static const double kLdltPredict = ldlt_bound("MSGPU_LDLT_PREDICT", 300.0);
+static const double kLdltAccept = ldlt_bound("MSGPU_LDLT_ACCEPT", 1000.0);
 
 static int tracelog_ldlt(const std::vector<MediumView>& mv, int nk, const int* midx, const msgpu_cplx* ks,
                          msgpu_cplx* out, int* flags) {
