@@ -76,9 +76,9 @@ enum {
     MSGPU_MIXED = 1,   /* complex64 LU on MFMA; FP64 re-evaluation of flagged cells  */
     MSGPU_FP32 = 2,    /* complex64 LU only (error reported, not corrected)          */
     MSGPU_MIXED_GATED = 3, /* complex64 LU; FP64 where its pivot growth could exceed tol */
-    MSGPU_FP64_LDLT = 4    /* complex128; pivot-free LDL^T where its growth stays small,
-                            * pivoted LU elsewhere (trace-log batches; complex128 in the
-                            * other entry points)                                      */
+    MSGPU_FP64_LDLT = 4    /* complex128; pivot-free LDL^T where its error estimate stays
+                            * small, pivoted LU elsewhere (trace-log batches; complex128
+                            * in the other entry points)                               */
 };
 
 /* ------------------------------------------------- 1. drop-in (reference ABI) */
@@ -181,8 +181,13 @@ int msgpu_tracelog_batch(const msgpu_medium* med, int nk, const msgpu_cplx* ks, 
  * bit.  Returns 0, or < 0 on error (message in msgpu_last_error). */
 int msgpu_tracelog_batch_noalloc(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, int mode);
 /* As msgpu_tracelog_batch in FP64 or FP32 (no re-evaluation), with the pivot statistics
- * stats[2l] = max_i log|U_ii|, stats[2l+1] = min_i log|U_ii| of cell l.  No counterpart in
- * the reference; a conditioning indicator for the caller. */
+ * stats[2l] = max_i log|U_ii|, stats[2l+1] = min_i log|U_ii| of cell l.  With
+ * MSGPU_FP64_LDLT every cell is the pivot-free LDL^T (scatterers in order of increasing
+ * distance from their centroid; no LU fallback) and stats[2l] = max |L|,
+ * stats[2l+1] = rho = sum_r sum_{k<r} |L_rk|^2 |D_k| / |D_r|, whose product with the
+ * double epsilon estimates the rounding error of log det (msgpu_tracelog_batch keeps the
+ * LDL^T of a cell when rho <= 1e7).  No counterpart in the reference; a conditioning
+ * indicator for the caller. */
 int msgpu_tracelog_stats(const msgpu_medium* med, int nk, const msgpu_cplx* ks, msgpu_cplx* out, double* stats,
                          int mode);
 /* d/dk log det M(k) = tr(M^-1 M') for nk wave numbers (ms_matrix_util.c:242-263). */

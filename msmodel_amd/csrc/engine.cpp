@@ -821,16 +821,20 @@ static ChunkGraph<T>* chunk_graph(hipStream_t st, int np, int B, bool rm, const 
 // the block's first 64 columns (diagonal block, rows), their update of its second 64
 // columns, the second 64 columns, then one K = 128 update of the lower tiles right of
 // the block (the complex128 GEMM's 3M product).  growth[b] receives the largest |L| of
-// matrix b (as the bits of a non-negative double).
-static void factor_ldlt(cx<double>* mats, int np, int nbatch, unsigned long long* growth, hipStream_t st) {
+// matrix b (as the bits of a non-negative double), rho[b] its error estimate
+// sum_r sum_{k<r} |L_rk|^2 |D_k| / |D_r| (acc: np doubles per matrix of workspace).
+static void factor_ldlt(cx<double>* mats, int np, int nbatch, unsigned long long* growth, double* acc, double* rho,
+                        hipStream_t st) {
     const long long ms = (long long)np * np;
     auto at = [&](int r, int c) { return mats + r + (size_t)c * np; };
     auto panel = [&](int k0) {
-        timed(0, 0.0, 0.0, st, [&] { launch_ldlt_diag<double>(mats, np, ms, k0, growth, nbatch, st); });
+        timed(0, 0.0, 0.0, st, [&] { launch_ldlt_diag<double>(mats, np, ms, k0, growth, acc, rho, nbatch, st); });
         if (np - k0 - NB > 0)
-            timed(1, 0.0, 0.0, st, [&] { launch_ldlt_rows<double>(mats, np, ms, k0, growth, nbatch, st); });
+            timed(1, 0.0, 0.0, st, [&] { launch_ldlt_rows<double>(mats, np, ms, k0, growth, acc, nbatch, st); });
     };
     HIPCHK(hipMemsetAsync(growth, 0, sizeof(unsigned long long) * nbatch, st));
+    HIPCHK(hipMemsetAsync(acc, 0, sizeof(double) * nbatch * np, st));
+    HIPCHK(hipMemsetAsync(rho, 0, sizeof(double) * nbatch, st));
     for (int k0 = 0; k0 < np; k0 += 2 * NB) {
         panel(k0);
         const int W1 = np - k0 - NB;
@@ -886,6 +890,7 @@ static void tracelog_ldlt_persist(const std::vector<MediumView>& meds, int nk, c
     const int nwg = std::max(1, std::min({nk, ncu * per_cu,
                                           (int)std::min<size_t>(1 << 20, batch_budget() / (ms * sizeof(cx<double>)))}));
     DBuf<cx<double>> ws((size_t)nwg * ms), dout(nk);
+    DBuf<double> accws((size_t)nwg * npmax), drho(nk);
     DBuf<unsigned long long> dgrow(nk);
     DBuf<int> counter(1);
     HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), L.st));
@@ -895,24 +900,30 @@ static void tracelog_ldlt_persist(const std::vector<MediumView>& meds, int nk, c
         flops += 4.0 * n * n * n / 3.0;
     }
     timed(6, flops, 0.0, L.st,
-          [&] { launch_ldlt_persist(D.items.p, nk, d3, ws.p, ms, nwg, counter.p, dout.p, dgrow.p, L.st); });
+          [&] {
+              launch_ldlt_persist(D.items.p, nk, d3, ws.p, ms, accws.p, npmax, nwg, counter.p, dout.p, dgrow.p, drho.p,
+                                  L.st);
+          });
     HIPCHK(hipGetLastError());
     std::vector<cx<double>> hout(nk);
     std::vector<unsigned long long> hgrow(nk);
+    std::vector<double> hrho(nk);
     HIPCHK(hipMemcpyAsync(hout.data(), dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, L.st));
     HIPCHK(hipMemcpyAsync(hgrow.data(), dgrow.p, sizeof(unsigned long long) * nk, hipMemcpyDeviceToHost, L.st));
+    HIPCHK(hipMemcpyAsync(hrho.data(), drho.p, sizeof(double) * nk, hipMemcpyDeviceToHost, L.st));
     L.sync();
     for (int q = 0; q < nk; q++) {
         out[order[q]] = hout[q];
         if (stats) {
             memcpy(&stats[2 * order[q]], &hgrow[q], sizeof(double));
-            stats[2 * order[q] + 1] = 0.0;
+            stats[2 * order[q] + 1] = hrho[q];
         }
     }
 }
 
 // ldlt (complex128 only): the pivot-free LDL^T instead of the LU; stats[2 l] receives the
-// growth max |L| of cell l (stats[2 l + 1] = 0).
+// growth max |L| of cell l, stats[2 l + 1] its error estimate
+// sum_r sum_{k<r} |L_rk|^2 |D_k| / |D_r| (eps times it estimates |d log det|).
 template <class T>
 static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int* midx, const cx<double>* ks,
                           cx<double>* out, double* stats, bool ldlt = false) {
@@ -928,11 +939,16 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
     DBuf<cx<double>> dout(nk);
     DBuf<double> dstats(2 * (size_t)nk);
     DBuf<unsigned long long> dgrow(ldlt ? nk : 1);
+    DBuf<double> drho(ldlt ? nk : 1);
+    size_t acc_elems = 1;      // the LDL^T's row sums: np doubles per matrix of a chunk
+    for (const Plan::Chunk& ch : P.chunks) acc_elems = std::max(acc_elems, (size_t)ch.count * ch.np);
     S.fork();
     const int nlane = std::min<int>(S.ns, (int)P.chunks.size());
     std::vector<DBuf<cx<T>>> mats(nlane);
     std::vector<DBuf<int>> ipiv(nlane), info(nlane), pmap(nlane);
+    std::vector<DBuf<double>> acc(ldlt ? nlane : 0);
     for (int i = 0; i < nlane; i++) {
+        if (ldlt) acc[i].alloc(acc_elems);
         mats[i].alloc(P.mat_elems);
         ipiv[i].alloc(P.piv_ints);
         info[i].alloc(P.max_b);
@@ -954,7 +970,7 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
                 timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st, [&] {
                     launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st, D.dim(ch.start, B));
                 });
-                factor_ldlt(mats[ln].p, np, B, dgrow.p + ch.start, st);
+                factor_ldlt(mats[ln].p, np, B, dgrow.p + ch.start, acc[ln].p, drho.p + ch.start, st);
                 timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {
                     launch_tracelog<T>(mats[ln].p, 0, D.items.p + ch.start, np, ms, nullptr, dout.p + ch.start,
                                        nullptr, B, st);
@@ -986,17 +1002,20 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
     std::vector<cx<double>> hout(nk);
     std::vector<double> hst(stats ? 2 * (size_t)nk : 0);
     std::vector<unsigned long long> hgrow(ldlt && stats ? nk : 0);
+    std::vector<double> hrho(ldlt && stats ? nk : 0);
     HIPCHK(hipMemcpyAsync(hout.data(), dout.p, sizeof(cx<double>) * nk, hipMemcpyDeviceToHost, S[0]));
     if (stats && !ldlt)
         HIPCHK(hipMemcpyAsync(hst.data(), dstats.p, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, S[0]));
-    if (stats && ldlt)
+    if (stats && ldlt) {
         HIPCHK(hipMemcpyAsync(hgrow.data(), dgrow.p, sizeof(unsigned long long) * nk, hipMemcpyDeviceToHost, S[0]));
+        HIPCHK(hipMemcpyAsync(hrho.data(), drho.p, sizeof(double) * nk, hipMemcpyDeviceToHost, S[0]));
+    }
     S.l[0].sync();
     for (size_t q = 0; q < hgrow.size(); q++) {
         double g;
         memcpy(&g, &hgrow[q], sizeof g);
         hst[2 * q] = g;
-        hst[2 * q + 1] = 0.0;
+        hst[2 * q + 1] = hrho[q];
     }
     for (hipEvent_t e : stag)
         if (e) (void)hipEventDestroy(e);

@@ -7,7 +7,8 @@
 //   the blocked LDL^T in 128-column blocks (per 64-column panel the diagonal block and
 //   the rows below it; the K = 64 update of the block's second panel; the K = 128 update
 //   of the lower tiles right of the block, on MFMA),
-//   sum_i log D_ii / N and the largest |L| of the item.
+//   sum_i log D_ii / N, the largest |L| of the item and its error estimate (ldlt_dev.h;
+//   the row sums in the workgroup's own Np doubles of accws).
 // These are the device functions of the per-phase launches (kern_assemble.hip,
 // kern_ldlt.hip, kern_lu.hip) in the same order, so each item's results are those of
 // the batched LDL^T bit for bit.  Items depend on nothing but themselves: no workgroup
@@ -35,13 +36,16 @@ static_assert(4 * 256 + 128 <= kPersistLds, "trace-log reduction");
 template <bool D3>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ldlt_persist_kernel(const MatItem* __restrict__ items, int nitems,
                                                            cx<double>* __restrict__ ws, long long wstride,
+                                                           double* __restrict__ accws, long long astride,
                                                            int* __restrict__ counter, cx<double>* __restrict__ out,
-                                                           unsigned long long* __restrict__ growth) {
+                                                           unsigned long long* __restrict__ growth,
+                                                           double* __restrict__ rho) {
     __shared__ __align__(16) double smem[kPersistLds];
     __shared__ double sred[4];
     __shared__ int s_item;
     const int tid = threadIdx.x;
     cx<double>* M = ws + (size_t)blockIdx.x * wstride;
+    double* acc = accws + (size_t)blockIdx.x * astride;
     auto at = [&](int r, int c, int np) { return M + r + (size_t)c * np; };
     for (;;) {
         if (tid == 0) s_item = atomicAdd(counter, 1);
@@ -50,6 +54,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
         if (it >= nitems) break;                           // uniform: every workgroup leaves here
         const MatItem item = items[it];
         const int n = item.n, np = (n + 63) / 64 * 64;
+        for (int i = tid; i < np; i += 256) acc[i] = 0.0;  // ordered before use by the barriers below
         {   // assembly, tile pair by tile pair
             auto sx = reinterpret_cast<double (*)[AT][4]>(smem);
             auto tile = reinterpret_cast<cx<double> (*)[AT + 1]>(smem + 2 * AT * 4);
@@ -65,9 +70,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
         // 64-column panels kp = 0, 64, ...: the diagonal block, then the rows below it; after
         // the first panel of a 128-column block, the K = 64 update of its second panel; after
         // the second, the K = 128 update of the lower tiles right of the block (3M)
-        double g = 0.0;
+        double g = 0.0, rs = 0.0;
         for (int kp = 0; kp < np; kp += 64) {
-            g = fmax(g, (double)ldlt_diag_block<double>(at(kp, kp, np), np, reinterpret_cast<cx<double> (*)[64]>(smem)));
+            double r = 0.0;
+            g = fmax(g, (double)ldlt_diag_block<double>(at(kp, kp, np), np, reinterpret_cast<cx<double> (*)[64]>(smem),
+                                                        acc + kp, r));
+            rs += r;
             __syncthreads();
             const int W = np - kp - 64;
             if (W <= 0) break;
@@ -78,7 +86,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
             // (a barrier per 64 rows: the triangle's reads are the same for every block of rows,
             // and hoisted out of the loop they would not fit in registers)
             for (int jb = 0; jb < (W + 63) / 64; jb++) {
-                g = fmax(g, (double)ldlt_rows_solve<double>(M, np, kp, jb, sL, sD));
+                g = fmax(g, (double)ldlt_rows_solve<double>(M, np, kp, jb, sL, sD, acc));
                 __syncthreads();
             }
             if ((kp & 127) == 0) {
@@ -98,7 +106,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
         tracelog_block<double>(M, n, np, nullptr, r, r + 256, r + 512, r + 768, reinterpret_cast<int*>(r + 1024),
                                out + it, nullptr);
         const double gm = group_max(g, sred);
-        if (tid == 0) growth[it] = (unsigned long long)__double_as_longlong(gm);
+        if (tid == 0) {
+            growth[it] = (unsigned long long)__double_as_longlong(gm);
+            rho[it] = rs;
+        }
         __syncthreads();                                   // s_item, smem and sred are reused
     }
 }
@@ -112,14 +123,15 @@ int persist_ldlt_blocks_per_cu(bool d3) {
     return e == hipSuccess ? nb : 0;
 }
 
-void launch_ldlt_persist(const MatItem* items, int nitems, bool d3, cx<double>* ws, long long wstride, int nwg,
-                         int* counter, cx<double>* out, unsigned long long* growth, hipStream_t st) {
+void launch_ldlt_persist(const MatItem* items, int nitems, bool d3, cx<double>* ws, long long wstride, double* accws,
+                         long long astride, int nwg, int* counter, cx<double>* out, unsigned long long* growth,
+                         double* rho, hipStream_t st) {
     if (d3)
-        hipLaunchKernelGGL(ldlt_persist_kernel<true>, dim3(nwg), dim3(256), 0, st, items, nitems, ws, wstride, counter,
-                           out, growth);
+        hipLaunchKernelGGL(ldlt_persist_kernel<true>, dim3(nwg), dim3(256), 0, st, items, nitems, ws, wstride, accws,
+                           astride, counter, out, growth, rho);
     else
-        hipLaunchKernelGGL(ldlt_persist_kernel<false>, dim3(nwg), dim3(256), 0, st, items, nitems, ws, wstride,
-                           counter, out, growth);
+        hipLaunchKernelGGL(ldlt_persist_kernel<false>, dim3(nwg), dim3(256), 0, st, items, nitems, ws, wstride, accws,
+                           astride, counter, out, growth, rho);
 }
 
 }  // namespace msg

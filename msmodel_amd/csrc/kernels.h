@@ -48,11 +48,13 @@ void launch_wave_points(const double* pos, int n, int d, const cx<double>* a, cx
 // Persistent trace-log by the pivot-free LDL^T (kern_persist.hip): nwg workgroups take
 // items[0..nitems) from *counter (0 at launch) and each runs an item's assembly,
 // factorisation and reduction in its own workspace matrix (ws + w * wstride, at least
-// Np^2 elements for the largest item); out[i] = sum log D_ii / N, growth[i] = the bits of
-// its largest |L|.  d3: every item is d = 3.
+// Np^2 elements for the largest item; accws + w * astride: Np doubles); out[i] =
+// sum log D_ii / N, growth[i] = the bits of its largest |L|, rho[i] = its error estimate
+// sum_r sum_{k<r} |L_rk|^2 |D_k| / |D_r|.  d3: every item is d = 3.
 int persist_ldlt_blocks_per_cu(bool d3);
-void launch_ldlt_persist(const MatItem* items, int nitems, bool d3, cx<double>* ws, long long wstride, int nwg,
-                         int* counter, cx<double>* out, unsigned long long* growth, hipStream_t st);
+void launch_ldlt_persist(const MatItem* items, int nitems, bool d3, cx<double>* ws, long long wstride, double* accws,
+                         long long astride, int nwg, int* counter, cx<double>* out, unsigned long long* growth,
+                         double* rho, hipStream_t st);
 // One request of the resident psi kernel (a slot of its ring in coherent pinned host
 // memory): the host fills the request fields and then publishes seq = index + 1; the
 // device writes psi and then done = index + 1.
@@ -116,13 +118,18 @@ template <class T>
 void launch_gemm_sub_lower(cx<T>* C, const cx<T>* A, const cx<T>* B, int m, int k, int lda, int ldb, int ldc,
                            long long sC, long long sA, long long sB, int nbatch, hipStream_t st);
 // Pivot-free LDL^T of symmetric matrices (kern_ldlt.hip), column-major, ld np: the 64 x 64
-// diagonal block at k0 becomes L11 (unit lower) and U11 = D11 L11^T; then the rows below
-// get L21 = A21 U11^-1 and the rows k0 .. k0+63 right of the block U12 = D11 L21^T.
-// growth[b] (bits of a non-negative double) = the largest |L| seen.
+// diagonal block at k0 becomes L11 (strictly lower part) and D11 (diagonal), zeros above;
+// then the rows below get L21 = A21 L11^-T D11^-1 and the rows k0 .. k0+63 right of the
+// block U12 = D11 L21^T.
+// growth[b] (bits of a non-negative double) = the largest |L| seen; acc (np doubles per
+// matrix, zero before the first block) and rho[b] (zero before the first block) carry the
+// error estimate rho[b] = sum_r sum_{k<r} |L_rk|^2 |D_k| / |D_r|.
 template <class T>
-void launch_ldlt_diag(cx<T>* A, int np, long long ms, int k0, unsigned long long* growth, int nbatch, hipStream_t st);
+void launch_ldlt_diag(cx<T>* A, int np, long long ms, int k0, unsigned long long* growth, const double* acc,
+                      double* rho, int nbatch, hipStream_t st);
 template <class T>
-void launch_ldlt_rows(cx<T>* A, int np, long long ms, int k0, unsigned long long* growth, int nbatch, hipStream_t st);
+void launch_ldlt_rows(cx<T>* A, int np, long long ms, int k0, unsigned long long* growth, double* acc, int nbatch,
+                      hipStream_t st);
 // n of matrix b is items[b].n when items is given, else n; ipiv null: no interchanges
 template <class T>
 void launch_tracelog(const cx<T>* A, int n, const MatItem* items, int np, long long mstride, const int* ipiv,

@@ -13,6 +13,17 @@ __device__ __forceinline__ T growth_of(cx<T> l) {
     return a == a ? a : T(INFINITY);      // NaN counts as unbounded growth
 }
 
+// The error estimate of the factorisation: rho_r = sum_{k<r} |L_rk|^2 |D_k| / |D_r| is the
+// diagonal of |L| |D| |L|^T relative to D_r, so eps rho_r bounds the relative rounding
+// error of D_r, and eps sum_r rho_r that of log det M (up to a modest factor).  The sums
+// run along with the factorisation: acc[r] (one double per row, zero at the start)
+// collects |L_rk|^2 |D_k| = |L_rk| |L_rk D_k| over the columns of the finished panels, and
+// the diagonal block of r adds its own columns and divides by |D_r|.
+template <class T>
+__device__ __forceinline__ T mag(cx<T> z) {
+    return sqrt(z.x * z.x + z.y * z.y);
+}
+
 // Largest g over the workgroup (256 threads, four waves), in thread 0; sred: 4 doubles
 // of shared memory.
 __device__ __forceinline__ double group_max(double g, double* sred) {
@@ -24,26 +35,33 @@ __device__ __forceinline__ double group_max(double g, double* sred) {
 }
 
 // The 64 x 64 diagonal block at M (its origin; ld np), held in registers: lane r of
-// wave w holds a(r, 16 w + j) in x[j].  Step c: the wave holding column c forms
-// l_r = a(r, c) / a(c, c) for r > c (the pivot from lane c by readlane) and passes the
-// column through LDS (sl: 2 x 64, double-buffered: one barrier per step); every wave then
-// updates a(r, j) -= l_r a(c, j) for r, j > c, with row c of its own columns read from
-// its lane c.  The arithmetic is the unblocked LDL^T's, element by element.  Returns this
-// thread's largest |l|.
+// wave w holds a(r, 16 w + j) in x[j]; only the lower triangle is factored.  Step c: the
+// wave holding column c passes y = a(:, c) through LDS (sl: 2 x 64, double-buffered: one
+// barrier per step); every wave forms l_r = y_r / y_c and updates
+// a(r, j) -= l_r y_j for r >= j > c, and the owner keeps l_r for r > c.  The update takes
+// y_j = a(j, c) from the column, not a(c, j) from the row: on a block whose two
+// triangles agree only to rounding, an LU-style update (row from the upper triangle)
+// makes L11 D11 L11^T drift from the block, and the rows below, formed against that
+// L11, lose accuracy in proportion to the growth of L (a numpy model of the blocked
+// scheme on the N = 1000 map: 1.3e-8 in log det at k = 11.35 - 1.08i against 5e-11 with
+// the symmetric update).  Returns this thread's largest |l|; rho receives sum_r rho_r
+// over the block's 64 rows in the first wave (acc: the block's rows, acc[r] for row r of
+// the block).
 template <class T>
-__device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, cx<T> (*sl)[64]) {
+__device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, cx<T> (*sl)[64],
+                                             const double* __restrict__ acc, double& rho) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // the block as its lower triangle defines it: the updates of the lower tiles leave the
-    // diagonal tile's two triangles equal only to rounding (L21 D U12 products), and
-    // factoring that slightly unsymmetric block while U12 is formed from L21 and D (which
-    // assumes a symmetric block) loses accuracy in proportion to the growth of L
+    // diagonal tile's two triangles equal only to rounding (L21 D U12 products); the upper
+    // triangle is neither read nor kept
     cx<T> x[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int col = 16 * wv + j;
-        x[j] = lane >= col ? M[(size_t)lane + (size_t)col * np] : M[(size_t)col + (size_t)lane * np];
+        x[j] = lane >= col ? M[(size_t)lane + (size_t)col * np] : cx<T>{T(0), T(0)};
     }
     T g = T(0);
+    double part = 0.0;                       // sum of |L_rc|^2 |D_c| over this wave's columns c
 #pragma unroll 1
     for (int cw = 0; cw < 4; cw++) {
 #pragma unroll
@@ -51,29 +69,38 @@ __device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, cx<T
             const int c = 16 * cw + jc;
             if (c == 63) break;
             const int buf = jc & 1;
-            if (wv == cw) {                  // column c of L11
-                const cx<T> piv{rdlane(x[jc].x, c), rdlane(x[jc].y, c)};
-                if (lane > c) {
-                    const cx<T> l = x[jc] * inv(piv);
-                    x[jc] = l;
-                    g = fmax(g, growth_of(l));
-                }
-                sl[buf][lane] = x[jc];
-            }
+            if (wv == cw) sl[buf][lane] = x[jc];          // column c before the division
             __syncthreads();
+            const cx<T> y = sl[buf][lane];
+            const cx<T> l = y * inv(sl[buf][c]);          // the same bits in every wave
+            if (wv == cw && lane > c) {                   // column c of L11
+                const T a = growth_of(l);
+                part += (double)(a * mag(y));
+                x[jc] = l;
+                g = fmax(g, a);
+            }
             if (16 * wv + 15 > c) {          // this wave has columns right of c
-                const cx<T> l = sl[buf][lane];
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
                     const int col = 16 * wv + j;
-                    const cx<T> u{rdlane(x[j].x, c), rdlane(x[j].y, c)};
-                    if (col > c && lane > c) x[j] = x[j] - l * u;
+                    if (col > c && lane >= col) x[j] = x[j] - l * sl[buf][col];
                 }
             }
         }
     }
+    __syncthreads();                         // the last step's reads of sl are done
+    double* sp = reinterpret_cast<double*>(&sl[0][0]);
+    sp[64 * wv + lane] = part;
 #pragma unroll
     for (int j = 0; j < 16; j++) M[(size_t)lane + (size_t)(16 * wv + j) * np] = x[j];
+    __syncthreads();                         // the parts and the block's stores are visible
+    if (wv == 0) {
+        const double s = acc[lane] + sp[lane] + sp[64 + lane] + sp[128 + lane] + sp[192 + lane];
+        double r = s / (double)mag(M[(size_t)lane + (size_t)lane * np]);
+        r = r == r ? r : INFINITY;
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off);
+        rho = r;
+    }
     return g;
 }
 
@@ -91,10 +118,11 @@ __device__ __forceinline__ void ldlt_rows_stage(const cx<T>* __restrict__ M, int
 }
 
 // Solve: the 64 rows k0 + 64 + 64 jb + [0, 64) (those below np are skipped), four lanes
-// per row.  Returns this thread's largest |l|.
+// per row.  Returns this thread's largest |l|; adds sum_k |L_rk|^2 |D_k| over the panel's
+// columns to acc[r] (acc: the matrix's rows).
 template <class T>
 __device__ __forceinline__ T ldlt_rows_solve(cx<T>* __restrict__ M, int np, int k0, int jb, const cx<T>* sL,
-                                             const cx<T>* sD) {
+                                             const cx<T>* sD, double* __restrict__ acc) {
     const int tid = threadIdx.x, qq = tid & 3;
     const int W = np - k0 - 64;
     const int j = jb * 64 + (tid >> 2);
@@ -110,6 +138,7 @@ __device__ __forceinline__ T ldlt_rows_solve(cx<T>* __restrict__ M, int np, int 
     asm volatile("" : "+v"(nb));
     lower_unit_solve_unrolled<T>(xv, PackedL<T>{sL}, nb, qq);
     T g = T(0);
+    double part = 0.0;
     if (live) {
 #pragma unroll
         for (int m = 0; m < 16; m++) {
@@ -117,9 +146,14 @@ __device__ __forceinline__ T ldlt_rows_solve(cx<T>* __restrict__ M, int np, int 
             const cx<T> l = xv[m] / sD[i];
             M[(size_t)(k0 + i) + row * np] = xv[m];                                   // U12 = D11 L21^T
             M[row + (size_t)(k0 + i) * np] = l;                                       // L21
-            g = fmax(g, growth_of(l));
+            const T a = growth_of(l);
+            part += (double)(a * mag(xv[m]));
+            g = fmax(g, a);
         }
     }
+    part += __shfl_xor(part, 1);             // the row's four lanes
+    part += __shfl_xor(part, 2);
+    if (live && qq == 0) acc[row] += part;
     return g;
 }
 
