@@ -49,9 +49,10 @@ __global__ __launch_bounds__(256) void ldlt_diag_kernel(cx<T>* __restrict__ A, i
     if (threadIdx.x == 0) rho[b] += r;
 }
 
-// Rows below the diagonal block, 64 per workgroup (ldlt_rows_stage, ldlt_rows_solve).
+// Rows below the diagonal block, 64 per workgroup (ldlt_rows_stage, ldlt_rows_solve);
+// three waves per SIMD (168 registers).
 template <class T>
-__global__ __launch_bounds__(256) void ldlt_rows_kernel(cx<T>* __restrict__ A, int np, long long ms, int k0,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ldlt_rows_kernel(cx<T>* __restrict__ A, int np, long long ms, int k0,
                                                         unsigned long long* __restrict__ growth,
                                                         double* __restrict__ acc) {
     __shared__ cx<T> sL[64 * 63 / 2];        // L11 strictly lower, packed by rows

@@ -132,11 +132,7 @@ __device__ __forceinline__ T ldlt_rows_solve(cx<T>* __restrict__ M, int np, int 
 #pragma unroll
     for (int m = 0; m < 16; m++)
         xv[m] = live ? M[row + (size_t)(k0 + 4 * m + qq) * np] : cx<T>{T(0), T(0)};
-    // (the block edge as a value the compiler cannot see through: known to be the constant
-    // 64, it schedules the unrolled solve into more registers than the file holds)
-    int nb = min(64, np - k0);
-    asm volatile("" : "+v"(nb));
-    lower_unit_solve_unrolled<T>(xv, PackedL<T>{sL}, nb, qq);
+    lower_unit_solve64<T, 1>(xv, PackedL<T>{sL}, qq);
     T g = T(0);
     double part = 0.0;
     if (live) {

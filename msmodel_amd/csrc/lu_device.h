@@ -254,4 +254,39 @@ __device__ __forceinline__ void lower_unit_solve_unrolled(cx<T> (&xv)[16], LA sL
     lower_steps<T>(xv, sL, nb, qq, std::make_integer_sequence<int, 64>{});
 }
 
+// The same for a full 64 x 64 triangle (no row limit), branch-free: one step per row I,
+// every register's rows below I updated with x - l y in four fused multiply-adds, and
+// the quad that straddles I masked by l = 0 (a branch there, or the row limit, made the
+// compiler keep the triangle's loads in scratch).  SB: a scheduling barrier every SB
+// steps (1: 168 registers, three waves per SIMD).
+// x - l y in four fused multiply-adds
+template <class T>
+__device__ __forceinline__ cx<T> fms4(cx<T> x, cx<T> l, cx<T> y) {
+    return {fma(-l.x, y.x, fma(l.y, y.y, x.x)), fma(-l.x, y.y, fma(-l.y, y.x, x.y))};
+}
+
+template <class T, int I, int SB, class LA>
+__device__ __forceinline__ void lower_step64(cx<T> (&xv)[16], LA sL, int qq) {
+    constexpr int mi = I >> 2, iq = I & 3;
+    const cx<T> xi{quad_bcast(xv[mi].x, iq), quad_bcast(xv[mi].y, iq)};
+    {   // the quad's rows at or above I take l = 0: x - 0 y = x, without a branch
+        const cx<T> l = sL(4 * mi + qq, I);
+        const bool below = qq > iq;
+        xv[mi] = fms4(xv[mi], cx<T>{below ? l.x : T(0), below ? l.y : T(0)}, xi);
+    }
+#pragma unroll
+    for (int m = mi + 1; m < 16; m++) xv[m] = fms4(xv[m], sL(4 * m + qq, I), xi);
+    if ((I % SB) == SB - 1) __builtin_amdgcn_sched_barrier(0);
+}
+
+template <class T, int SB, class LA, int... Is>
+__device__ __forceinline__ void lower_steps64(cx<T> (&xv)[16], LA sL, int qq, std::integer_sequence<int, Is...>) {
+    (lower_step64<T, Is, SB>(xv, sL, qq), ...);
+}
+
+template <class T, int SB = 4, class LA>
+__device__ __forceinline__ void lower_unit_solve64(cx<T> (&xv)[16], LA sL, int qq) {
+    lower_steps64<T, SB>(xv, sL, qq, std::make_integer_sequence<int, 64>{});
+}
+
 }  // namespace msg
