@@ -12,7 +12,7 @@ namespace msg {
 // elements in flight per workgroup).  Right tiles then get U12 = L11^-1 A12, 64
 // columns at a time against L11 staged once in LDS.
 template <class T, int TW, int RW, bool RM>
-__global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, int np, long long mstride,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void swap_trsm_kernel(cx<T>* __restrict__ A, int np, long long mstride,
                                                         const int* __restrict__ pmap_all, int k0, int nb,
                                                         SwapRange sr) {
     const int nleft_t = sr.nleft_t;
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void swap_trsm_kernel(cx<T>* __restrict__ A, i
             const int i = 4 * m + qq;
             xv[m] = (live && i < nb) ? M[at<RM>(k0 + i, col, np)] : cx<T>{T(0), T(0)};
         }
-        lower_unit_solve_unrolled<T>(xv, PackedL<T>{sL}, nb, qq);
+        lower_unit_solve64<T, 1>(xv, PackedL<T>{sL}, qq);          // nb = 64: every panel is full
         if (live) {
 #pragma unroll
             for (int m = 0; m < 16; m++) {
