@@ -43,10 +43,13 @@ __device__ __forceinline__ cx<T> neg_green3_s(cx<T> k, T s) {
 constexpr int AT = 32;                 // tile edge (256 threads: AT / (256 / AT) elements each)
 constexpr int ARW = 256 / AT;           // thread rows of a tile
 
+// lower: the tile below the diagonal only (J, I) for I < J (the pivot-free LDL^T reads no
+// other); the diagonal tiles in full either way.
 template <class T, bool D3>
 __device__ __forceinline__ void assemble_pair(const double* __restrict__ pos, int n, int d, int np, cx<double> kd,
                                               cx<double> dg, cx<double> iz, int kind, cx<T>* __restrict__ M,
-                                              int I, int J, double (*sx)[AT][4], cx<T> (*tile)[AT + 1]) {
+                                              int I, int J, double (*sx)[AT][4], cx<T> (*tile)[AT + 1],
+                                              bool lower = false) {
     const int tid = threadIdx.x;
     if (tid < 2 * AT) {
         int side = tid / AT, a = tid % AT;
@@ -90,7 +93,7 @@ __device__ __forceinline__ void assemble_pair(const double* __restrict__ pos, in
                 v = neg_green<T>(d, k, r);
             }
         }
-        M[(size_t)gi + (size_t)gj * np] = v;
+        if (!lower || I == J) M[(size_t)gi + (size_t)gj * np] = v;
         tile[jj][ii] = v;
     }
     if (I == J) return;

@@ -967,8 +967,10 @@ static void tracelog_impl(const std::vector<MediumView>& meds, int nk, const int
         if constexpr (std::is_same<T, double>::value) {
             if (ldlt) {
                 if ((int)c + 1 < nlane) HIPCHK(hipEventRecord(stag[c], st));
-                timed(3, 0.0, (double)B * ms * sizeof(cx<T>), st, [&] {
-                    launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st, D.dim(ch.start, B));
+                // the lower triangle and the 32 x 32 diagonal tiles: np (np + 32) / 2 elements
+                timed(3, 0.0, (double)B * 0.5 * np * (np + 32) * sizeof(cx<T>), st, [&] {
+                    launch_assemble_items<T>(D.items.p + ch.start, np, 0, mats[ln].p, ms, B, st, D.dim(ch.start, B),
+                                             true);
                 });
                 factor_ldlt(mats[ln].p, np, B, dgrow.p + ch.start, acc[ln].p, drho.p + ch.start, st);
                 timed(4, 0.0, (double)B * np * sizeof(cx<T>), st, [&] {

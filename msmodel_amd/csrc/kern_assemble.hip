@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
                                                        const cx<double>* __restrict__ diag,
                                                        const cx<double>* __restrict__ izero, int kind,
                                                        cx<T>* __restrict__ out, long long mstride,
-                                                       const MatItem* __restrict__ items) {
+                                                       const MatItem* __restrict__ items, int lower) {
     const int b = blockIdx.y;
     cx<double> kd, dg;
     if (items) {            // per-matrix medium (a batch over several media of one padded size)
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const double* __restrict_
     __shared__ cx<T> tile[AT][AT + 1];
     cx<double> iz{1.0, 0.0};
     if (kind == 2) iz = izero[b];
-    assemble_pair<T, D3>(pos, n, d, np, kd, dg, iz, kind, out + (size_t)b * mstride, I, J, sx, tile);
+    assemble_pair<T, D3>(pos, n, d, np, kd, dg, iz, kind, out + (size_t)b * mstride, I, J, sx, tile, lower != 0);
 }
 
 template <class T>
@@ -87,28 +87,30 @@ void launch_assemble(const double* pos, int n, int d, int np, const cx<double>* 
     dim3 grid(nt * (nt + 1) / 2, nbatch);
     if (d == 3 && kind == 0)
         hipLaunchKernelGGL((assemble_kernel<T, true>), grid, dim3(256), 0, st, pos, n, d, np, ks, diag, izero, kind,
-                           out, mstride, (const MatItem*)nullptr);
+                           out, mstride, (const MatItem*)nullptr, 0);
     else
         hipLaunchKernelGGL((assemble_kernel<T, false>), grid, dim3(256), 0, st, pos, n, d, np, ks, diag, izero, kind,
-                           out, mstride, (const MatItem*)nullptr);
+                           out, mstride, (const MatItem*)nullptr, 0);
 }
 
 template <class T>
 void launch_assemble_items(const MatItem* items, int np, int kind, cx<T>* out, long long mstride, int nbatch,
-                           hipStream_t st, int dall) {
+                           hipStream_t st, int dall, bool lower) {
     const int nt = np / AT;
     dim3 grid(nt * (nt + 1) / 2, nbatch);
     if (dall == 3 && kind == 0)
         hipLaunchKernelGGL((assemble_kernel<T, true>), grid, dim3(256), 0, st, (const double*)nullptr, 0, 0, np,
                            (const cx<double>*)nullptr, (const cx<double>*)nullptr, (const cx<double>*)nullptr, kind,
-                           out, mstride, items);
+                           out, mstride, items, lower ? 1 : 0);
     else
         hipLaunchKernelGGL((assemble_kernel<T, false>), grid, dim3(256), 0, st, (const double*)nullptr, 0, 0, np,
                            (const cx<double>*)nullptr, (const cx<double>*)nullptr, (const cx<double>*)nullptr, kind,
-                           out, mstride, items);
+                           out, mstride, items, lower ? 1 : 0);
 }
-template void launch_assemble_items<double>(const MatItem*, int, int, cx<double>*, long long, int, hipStream_t, int);
-template void launch_assemble_items<float>(const MatItem*, int, int, cx<float>*, long long, int, hipStream_t, int);
+template void launch_assemble_items<double>(const MatItem*, int, int, cx<double>*, long long, int, hipStream_t, int,
+                                            bool);
+template void launch_assemble_items<float>(const MatItem*, int, int, cx<float>*, long long, int, hipStream_t, int,
+                                           bool);
 template void launch_assemble<double>(const double*, int, int, int, const cx<double>*, const cx<double>*,
                                       const cx<double>*, int, cx<double>*, long long, int, hipStream_t);
 template void launch_assemble<float>(const double*, int, int, int, const cx<double>*, const cx<double>*,

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
                 int I, J;
                 tile_pair(t, nt, I, J);
                 assemble_pair<double, D3>(item.pos, n, item.d, np, item.k, item.diag0, cx<double>{1.0, 0.0}, 0, M,
-                                          I, J, sx, tile);
+                                          I, J, sx, tile, true);
                 __syncthreads();
             }
         }

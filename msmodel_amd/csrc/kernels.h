@@ -23,10 +23,11 @@ void launch_item_diag(MatItem* items, int cnt, hipStream_t st);
 // phi[b np + i] = e^{i k_b x_i0} for i < n_b, 0 below np (the plane wave along 1x)
 void launch_plane_items(const MatItem* items, int np, cx<double>* phi, int nbatch, hipStream_t st);
 // kind 0: M, kind 1: dM/dk, from items[0..nbatch); dall: the dimension d shared by every
-// item, or 0 when they differ (dall = 3 with kind 0 launches the kernel specialised for d = 3)
+// item, or 0 when they differ (dall = 3 with kind 0 launches the kernel specialised for d = 3);
+// lower: the lower triangle only (the diagonal tiles in full; the rest is left as it was)
 template <class T>
 void launch_assemble_items(const MatItem* items, int np, int kind, cx<T>* out, long long mstride, int nbatch,
-                           hipStream_t st, int dall = 0);
+                           hipStream_t st, int dall = 0, bool lower = false);
 // out[b] = tr(M_b^-1 M_b') from V_b, the row-major factor's solve against the identity
 // (M^-1 = P V, perm from launch_build_perm), with M' generated on the fly; inv (np ints
 // per matrix) receives the inverse permutation, partial dk_contract_partials(np) sums per
