@@ -26,26 +26,31 @@ namespace msg {
 
 namespace {
 
-// Largest value over the workgroup, folded into growth[b] (the bits of a non-negative
-// double: atomicMax on the integer order).
+// Largest value over the workgroup (NW waves), folded into growth[b] (the bits of a
+// non-negative double: atomicMax on the integer order).
+template <int NW = 4>
 __device__ __forceinline__ void fold_growth(double g, unsigned long long* growth, int b, double* sred) {
-    const double m = group_max(g, sred);
+    const double m = group_max<NW>(g, sred);
     if (threadIdx.x == 0) atomicMax(growth + b, (unsigned long long)__double_as_longlong(m));
 }
 
-// One workgroup per matrix: the diagonal block at k0 (ldlt_diag_block).  The blocks of a
-// matrix are factored by successive launches, so rho[b] takes a plain sum.
+// One workgroup per matrix: the diagonal block at k0 (ldlt_diag_block), eight waves of
+// eight columns: a launch has one workgroup per matrix, and with four waves of sixteen
+// columns each SIMD held a single wave of dependent steps.  The blocks of a matrix are
+// factored by successive launches, so rho[b] takes a plain sum.
+constexpr int kDiagWaves = 8;
 template <class T>
-__global__ __launch_bounds__(256) void ldlt_diag_kernel(cx<T>* __restrict__ A, int np, long long ms, int k0,
-                                                        unsigned long long* __restrict__ growth,
-                                                        const double* __restrict__ acc, double* __restrict__ rho) {
-    __shared__ cx<T> sl[2][64];
-    __shared__ double sred[4];
+__global__ __launch_bounds__(64 * kDiagWaves) void ldlt_diag_kernel(cx<T>* __restrict__ A, int np, long long ms,
+                                                                    int k0, unsigned long long* __restrict__ growth,
+                                                                    const double* __restrict__ acc,
+                                                                    double* __restrict__ rho) {
+    __shared__ __align__(16) double sbuf[512];
+    __shared__ double sred[kDiagWaves];
     const int b = blockIdx.x;
     double r = 0.0;
-    const T g = ldlt_diag_block<T>(A + (size_t)b * ms + (size_t)k0 + (size_t)k0 * np, np, sl,
-                                   acc + (size_t)b * np + k0, r);
-    fold_growth((double)g, growth, b, sred);
+    const T g = ldlt_diag_block<T, kDiagWaves>(A + (size_t)b * ms + (size_t)k0 + (size_t)k0 * np, np, sbuf,
+                                               acc + (size_t)b * np + k0, r);
+    fold_growth<kDiagWaves>((double)g, growth, b, sred);
     if (threadIdx.x == 0) rho[b] += r;
 }
 
@@ -71,7 +76,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
 template <class T>
 void launch_ldlt_diag(cx<T>* A, int np, long long ms, int k0, unsigned long long* growth, const double* acc,
                       double* rho, int nbatch, hipStream_t st) {
-    hipLaunchKernelGGL(ldlt_diag_kernel<T>, dim3(nbatch), dim3(256), 0, st, A, np, ms, k0, growth, acc, rho);
+    hipLaunchKernelGGL(ldlt_diag_kernel<T>, dim3(nbatch), dim3(64 * kDiagWaves), 0, st, A, np, ms, k0, growth, acc,
+                       rho);
 }
 
 template <class T>

@@ -30,6 +30,7 @@ static_assert(2 * AT * 4 + AT * (AT + 1) * 2 <= kPersistLds, "assembly staging")
 static_assert((64 * 63 / 2 + 64) * 2 <= kPersistLds, "rows staging");
 static_assert(KC3 * (2 * GT + PADA) + GT * (2 * KC3 + PADB) <= kPersistLds, "3M staging");
 static_assert(4 * 256 + 128 <= kPersistLds, "trace-log reduction");
+static_assert(512 <= kPersistLds, "diagonal block");
 
 // Three workgroups per CU (the register budget of the 3M tile, as for the batched GEMM),
 // so that one workgroup's latency-bound phases overlap another's GEMM tiles.
@@ -73,8 +74,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
         double g = 0.0, rs = 0.0;
         for (int kp = 0; kp < np; kp += 64) {
             double r = 0.0;
-            g = fmax(g, (double)ldlt_diag_block<double>(at(kp, kp, np), np, reinterpret_cast<cx<double> (*)[64]>(smem),
-                                                        acc + kp, r));
+            g = fmax(g, (double)ldlt_diag_block<double, 4>(at(kp, kp, np), np, smem, acc + kp, r));
             rs += r;
             __syncthreads();
             const int W = np - kp - 64;

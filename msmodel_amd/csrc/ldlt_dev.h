@@ -24,49 +24,60 @@ __device__ __forceinline__ T mag(cx<T> z) {
     return sqrt(z.x * z.x + z.y * z.y);
 }
 
-// Largest g over the workgroup (256 threads, four waves), in thread 0; sred: 4 doubles
-// of shared memory.
+// Largest g over the workgroup (NW waves of 64), in every thread; sred: NW doubles of
+// shared memory.
+template <int NW = 4>
 __device__ __forceinline__ double group_max(double g, double* sred) {
     for (int off = 32; off > 0; off >>= 1) g = fmax(g, __shfl_xor(g, off));
     const int tid = threadIdx.x;
     if ((tid & 63) == 0) sred[tid >> 6] = g;
     __syncthreads();
-    return fmax(fmax(sred[0], sred[1]), fmax(sred[2], sred[3]));
+    double m = sred[0];
+#pragma unroll
+    for (int w = 1; w < NW; w++) m = fmax(m, sred[w]);
+    return m;
 }
 
-// The 64 x 64 diagonal block at M (its origin; ld np), held in registers: lane r of
-// wave w holds a(r, 16 w + j) in x[j]; only the lower triangle is factored.  Step c: the
-// wave holding column c passes y = a(:, c) through LDS (sl: 2 x 64, double-buffered: one
-// barrier per step); every wave forms l_r = y_r / y_c and updates
+// The 64 x 64 diagonal block at M (its origin; ld np), held in registers by NW waves of
+// CW = 64 / NW columns: lane r of wave w holds a(r, CW w + j) in x[j]; only the lower
+// triangle is factored.  Step c: the wave holding column c passes y = a(:, c) through LDS
+// (double-buffered: one barrier per step); every wave forms l_r = y_r / y_c and updates
 // a(r, j) -= l_r y_j for r >= j > c, and the owner keeps l_r for r > c.  The update takes
 // y_j = a(j, c) from the column, not a(c, j) from the row: on a block whose two
 // triangles agree only to rounding, an LU-style update (row from the upper triangle)
 // makes L11 D11 L11^T drift from the block, and the rows below, formed against that
 // L11, lose accuracy in proportion to the growth of L (a numpy model of the blocked
 // scheme on the N = 1000 map: 1.3e-8 in log det at k = 11.35 - 1.08i against 5e-11 with
-// the symmetric update).  Returns this thread's largest |l|; rho receives sum_r rho_r
-// over the block's 64 rows in the first wave (acc: the block's rows, acc[r] for row r of
-// the block).
-template <class T>
-__device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, cx<T> (*sl)[64],
+// the symmetric update).  Every element sees the same operations in the same order for
+// any NW, and rho's partial sums are kept per 8 columns, so NW changes no bit.  Returns
+// this thread's largest |l|; rho receives sum_r rho_r over the block's 64 rows in the
+// first wave (acc: the block's rows, acc[r] for row r of the block); sbuf: 512 doubles of
+// shared memory.
+template <class T, int NW = 4>
+__device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, double* sbuf,
                                              const double* __restrict__ acc, double& rho) {
+    constexpr int CW = 64 / NW, NP = CW / 8;
+    static_assert(NW == 4 || NW == 8, "four or eight waves");
+    auto sl = reinterpret_cast<cx<T> (*)[64]>(sbuf);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // the block as its lower triangle defines it: the updates of the lower tiles leave the
     // diagonal tile's two triangles equal only to rounding (L21 D U12 products); the upper
     // triangle is neither read nor kept
-    cx<T> x[16];
+    cx<T> x[CW];
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int col = 16 * wv + j;
+    for (int j = 0; j < CW; j++) {
+        const int col = CW * wv + j;
         x[j] = lane >= col ? M[(size_t)lane + (size_t)col * np] : cx<T>{T(0), T(0)};
     }
     T g = T(0);
-    double part = 0.0;                       // sum of |L_rc|^2 |D_c| over this wave's columns c
-#pragma unroll 1
-    for (int cw = 0; cw < 4; cw++) {
+    double part[NP];                         // sums of |L_rc|^2 |D_c| over 8 columns c each
 #pragma unroll
-        for (int jc = 0; jc < 16; jc++) {
-            const int c = 16 * cw + jc;
+    for (int q = 0; q < NP; q++) part[q] = 0.0;
+#pragma unroll 1
+    for (int cw = 0; cw < NW; cw++) {
+#pragma unroll
+        for (int jc = 0; jc < CW; jc++) {
+            const int c = CW * cw + jc;
             if (c == 63) break;
             const int buf = jc & 1;
             if (wv == cw) sl[buf][lane] = x[jc];          // column c before the division
@@ -75,27 +86,30 @@ __device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, cx<T
             const cx<T> l = y * inv(sl[buf][c]);          // the same bits in every wave
             if (wv == cw && lane > c) {                   // column c of L11
                 const T a = growth_of(l);
-                part += (double)(a * mag(y));
+                part[jc / 8] += (double)(a * mag(y));
                 x[jc] = l;
                 g = fmax(g, a);
             }
-            if (16 * wv + 15 > c) {          // this wave has columns right of c
+            if (CW * wv + CW - 1 > c) {      // this wave has columns right of c
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const int col = 16 * wv + j;
+                for (int j = 0; j < CW; j++) {
+                    const int col = CW * wv + j;
                     if (col > c && lane >= col) x[j] = x[j] - l * sl[buf][col];
                 }
             }
         }
     }
     __syncthreads();                         // the last step's reads of sl are done
-    double* sp = reinterpret_cast<double*>(&sl[0][0]);
-    sp[64 * wv + lane] = part;
+    double* sp = sbuf;
 #pragma unroll
-    for (int j = 0; j < 16; j++) M[(size_t)lane + (size_t)(16 * wv + j) * np] = x[j];
+    for (int q = 0; q < NP; q++) sp[64 * (NP * wv + q) + lane] = part[q];
+#pragma unroll
+    for (int j = 0; j < CW; j++) M[(size_t)lane + (size_t)(CW * wv + j) * np] = x[j];
     __syncthreads();                         // the parts and the block's stores are visible
     if (wv == 0) {
-        const double s = acc[lane] + sp[lane] + sp[64 + lane] + sp[128 + lane] + sp[192 + lane];
+        double s = acc[lane];
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += sp[64 * q + lane];
         double r = s / (double)mag(M[(size_t)lane + (size_t)lane * np]);
         r = r == r ? r : INFINITY;
         for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off);
