@@ -217,54 +217,25 @@ __device__ __forceinline__ void upper_solve(cx<T> (&xv)[16], LA sL, int qq, int 
     }
 }
 
-// The unit-diagonal forward solve with the 64 steps unrolled: same lanes and registers,
-// and each row receives its updates in the same order, so the results are the same bits.
-// Every register index, step i and row offset is then a compile-time constant, and an
-// element sL(4m+qq, i) of the packed triangle is read at a per-lane base of register m
-// plus a constant offset, with no address arithmetic per access (the rolled loop above,
-// whose register file rotates, issues several times the multiply-adds in instructions,
-// mostly for those addresses).  Called by whole waves.  Kernels that run several solves
-// over the same triangle keep the rolled form (unrolled, they spill); so do the solves
-// with a stored diagonal, whose unrolled form the compiler neither unrolls nor (forced)
-// keeps in registers.
-// One step i of it; the steps are expanded from a parameter pack, so each is unrolled
-// by construction.
-template <class T, int I, class LA>
-__device__ __forceinline__ void lower_step(cx<T> (&xv)[16], LA sL, int nb, int qq) {
-    constexpr int mi = I >> 2, iq = I & 3;
-    const cx<T> xi{quad_bcast(xv[mi].x, iq), quad_bcast(xv[mi].y, iq)};
-#pragma unroll
-    for (int m = mi; m < 16; m++) {
-        const int row = 4 * m + qq;
-        if (row > I && row < nb) xv[m] = xv[m] - sL(row, I) * xi;
-    }
-    // every 4 steps: later steps' loads may not move above this point (unbounded, the
-    // scheduler keeps hundreds of them in flight and spills)
-    if (iq == 3) __builtin_amdgcn_sched_barrier(0);
-}
-
-template <class T, class LA, int... Is>
-__device__ __forceinline__ void lower_steps(cx<T> (&xv)[16], LA sL, int nb, int qq, std::integer_sequence<int, Is...>) {
-    (lower_step<T, Is>(xv, sL, nb, qq), ...);
-}
-
-// Unit-diagonal forward substitution, unrolled (see above).
-template <class T, class LA>
-__device__ __forceinline__ void lower_unit_solve_unrolled(cx<T> (&xv)[16], LA sL, int nb, int qq) {
-    lower_steps<T>(xv, sL, nb, qq, std::make_integer_sequence<int, 64>{});
-}
-
-// The same for a full 64 x 64 triangle (no row limit), branch-free: one step per row I,
-// every register's rows below I updated with x - l y in four fused multiply-adds, and
-// the quad that straddles I masked by l = 0 (a branch there, or the row limit, made the
-// compiler keep the triangle's loads in scratch).  SB: a scheduling barrier every SB
-// steps (1: 168 registers, three waves per SIMD).
 // x - l y in four fused multiply-adds
 template <class T>
 __device__ __forceinline__ cx<T> fms4(cx<T> x, cx<T> l, cx<T> y) {
     return {fma(-l.x, y.x, fma(l.y, y.y, x.x)), fma(-l.x, y.y, fma(-l.y, y.x, x.y))};
 }
 
+// The unit-diagonal forward solve of a full 64 x 64 triangle with the 64 steps unrolled
+// (expanded from a parameter pack): same lanes and registers as the rolled form above.
+// Every register index, step I and row offset is a compile-time constant, so an element
+// sL(4m+qq, I) of the packed triangle is read at a per-lane base of register m plus a
+// constant offset, with no address arithmetic per access (the rolled loop, whose register
+// file rotates, issues several times the multiply-adds in instructions, mostly for those
+// addresses).  Branch-free: each step updates every register's rows below I with x - l y
+// in four fused multiply-adds, and the quad that straddles I is masked by l = 0 (a branch
+// there, or a runtime row limit, made the compiler keep the triangle's loads in scratch).
+// SB: a scheduling barrier every SB steps (1: 168 registers, three waves per SIMD, in the
+// kernels that use it).  Kernels that run several solves over the same triangle keep the
+// rolled form (unrolled, they spill); so do the solves with a stored diagonal.  Called by
+// whole waves.
 template <class T, int I, int SB, class LA>
 __device__ __forceinline__ void lower_step64(cx<T> (&xv)[16], LA sL, int qq) {
     constexpr int mi = I >> 2, iq = I & 3;
