@@ -75,6 +75,25 @@ def test_sizes_and_dimensions(G, d, n, alpha):
             assert abs(wrapped(n * (t.imag - want.imag))) <= 1e-9
 
 
+@pytest.mark.parametrize("d,n,model,p", [(3, 65, 1, (0.2,)), (2, 120, 2, (5.0, -0.5)), (2, 90, 3, (0.0,)),
+                                         (3, 2, 0, (0.1,)), (2, 3, 0, (0.1,)), (4, 65, 2, (5.0, -0.5))])
+def test_scattering_models_and_tiny_sizes(G, d, n, model, p):
+    """The soft-sphere, resonant and maximal-coupling inverse amplitudes (the diagonal of M,
+    ms_matrix_util.c:36-41) and the smallest media, N = 2, 3 (medium_util.c:37-40 rejects
+    N < 2): against the LU map, at N <= 65 also against the oracle's LAPACK-style LU."""
+    med = G.Medium(d, n, model, p).fill(11)
+    ks = kmap(5, 3, 2, 8, -0.6, 0)
+    lu, _ = G.tracelog_batch(med, ks, G.FP64)
+    ldl, flags = G.tracelog_batch(med, ks, G.FP64_LDLT)
+    check(ldl, flags, lu, n)
+    if n <= 65:
+        pos = O.fill_medium(d, n, 11)
+        for k, t in zip(ks, ldl):
+            want = O.tracelog(O.build_matrix(pos, k, model, p))
+            assert abs(n * (t.real - want.real)) <= 1e-9, (k, t, want)
+            assert abs(wrapped(n * (t.imag - want.imag))) <= 1e-9, (k, t, want)
+
+
 def test_multi_medium(G):
     """Cells of several media and sizes in one call, as the LU map does them."""
     meds = [G.Medium(3, n, G.HARDSPHERE, (0.1,)).fill(s) for n, s in ((300, 1), (200, 2), (64, 3))]
