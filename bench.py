@@ -500,10 +500,9 @@ def main():
     n, nk = args.n, args.grid * args.grid
     med = G.Medium(3, n, G.HARDSPHERE, (0.05,)).fill(1)
     ks = kgrid(args.grid, args.grid, 5.0, 30.0, -2.0, 0.0)
-    # set-up, untimed: the autotuner picks launch shapes for this N in both precisions
-    # launch shapes: the tuning table loaded at initialisation (the one shipped beside the
-    # library, or MSGPU_TUNE_FILE), or with --tune the autotuner run here, untimed, saved
-    # to --tune-file when given
+    # launch shapes (set-up, untimed): the tuning table loaded at initialisation (the one
+    # shipped beside the library, or MSGPU_TUNE_FILE), or with --tune the autotuner run
+    # here for this N in both precisions, saved to --tune-file when given
     if args.tune:
         # (batches of 2048 matrices up to N = 1024: enough chunks for the stream count to show)
         items = lambda m: 2048 if m <= 1024 else 512
@@ -515,7 +514,9 @@ def main():
         if args.tune_file:
             G.tune_save(args.tune_file)
     else:
-        tuned = {"table": G.tuning_table_path()}
+        tp = G.tuning_table_path()
+        root = os.path.dirname(os.path.abspath(__file__))
+        tuned = {"table": os.path.relpath(tp, root) if tp and os.path.abspath(tp).startswith(root + os.sep) else tp}
     progress("launch shapes", tuned)
 
     if args.roofline_only:
