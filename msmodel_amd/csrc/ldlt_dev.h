@@ -24,6 +24,16 @@ __device__ __forceinline__ T mag(cx<T> z) {
     return sqrt(z.x * z.x + z.y * z.y);
 }
 
+// v as held by lane src of the wave (src the same in every lane).
+__device__ __forceinline__ double lane_value(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)b, src), hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_value(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
 // Largest g over the workgroup (NW waves of 64), in every thread; sred: NW doubles of
 // shared memory.
 template <int NW = 4>
@@ -58,7 +68,11 @@ __device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, doub
                                              const double* __restrict__ acc, double& rho) {
     constexpr int CW = 64 / NW, NP = CW / 8;
     static_assert(NW == 4 || NW == 8, "four or eight waves");
-    auto sl = reinterpret_cast<cx<T> (*)[64]>(sbuf);
+    static_assert(4 * 64 * sizeof(cx<T>) <= 512 * sizeof(double), "two buffers of y and l");
+    // step c: the wave owning column c divides it by the pivot and publishes the column
+    // before (sy) and after (sl) the division; the other waves only read them
+    auto sy = reinterpret_cast<cx<T> (*)[64]>(sbuf);
+    auto sl = sy + 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // the block as its lower triangle defines it: the updates of the lower tiles leave the
     // diagonal tile's two triangles equal only to rounding (L21 D U12 products); the upper
@@ -80,21 +94,26 @@ __device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, doub
             const int c = CW * cw + jc;
             if (c == 63) break;
             const int buf = jc & 1;
-            if (wv == cw) sl[buf][lane] = x[jc];          // column c before the division
-            __syncthreads();
-            const cx<T> y = sl[buf][lane];
-            const cx<T> l = y * inv(sl[buf][c]);          // the same bits in every wave
-            if (wv == cw && lane > c) {                   // column c of L11
-                const T a = growth_of(l);
-                part[jc / 8] += (double)(a * mag(y));
-                x[jc] = l;
-                g = fmax(g, a);
+            if (wv == cw) {
+                const cx<T> y = x[jc];                    // column c before the division
+                const cx<T> pv = {lane_value(y.x, c), lane_value(y.y, c)};
+                const cx<T> l = y * inv(pv);
+                sy[buf][lane] = y;
+                sl[buf][lane] = l;
+                if (lane > c) {                           // column c of L11
+                    const T a = growth_of(l);
+                    part[jc / 8] += (double)(a * mag(y));
+                    x[jc] = l;
+                    g = fmax(g, a);
+                }
             }
+            __syncthreads();
             if (CW * wv + CW - 1 > c) {      // this wave has columns right of c
+                const cx<T> l = sl[buf][lane];
 #pragma unroll
                 for (int j = 0; j < CW; j++) {
                     const int col = CW * wv + j;
-                    if (col > c && lane >= col) x[j] = x[j] - l * sl[buf][col];
+                    if (col > c && lane >= col) x[j] = x[j] - l * sy[buf][col];
                 }
             }
         }
