@@ -50,9 +50,9 @@ __device__ __forceinline__ double group_max(double g, double* sred) {
 
 // The 64 x 64 diagonal block at M (its origin; ld np), held in registers by NW waves of
 // CW = 64 / NW columns: lane r of wave w holds a(r, CW w + j) in x[j]; only the lower
-// triangle is factored.  Step c: the wave holding column c passes y = a(:, c) through LDS
-// (double-buffered: one barrier per step); every wave forms l_r = y_r / y_c and updates
-// a(r, j) -= l_r y_j for r >= j > c, and the owner keeps l_r for r > c.  The update takes
+// triangle is factored.  Step c: the wave holding column c forms l_r = y_r / y_c from
+// y = a(:, c), keeps l_r for r > c and passes y and l through LDS (double-buffered: one
+// barrier per step); every wave updates a(r, j) -= l_r y_j for r >= j > c.  The update takes
 // y_j = a(j, c) from the column, not a(c, j) from the row: on a block whose two
 // triangles agree only to rounding, an LU-style update (row from the upper triangle)
 // makes L11 D11 L11^T drift from the block, and the rows below, formed against that
@@ -140,13 +140,14 @@ __device__ __forceinline__ T ldlt_diag_block(cx<T>* __restrict__ M, int np, doub
 // Rows below the diagonal block at k0, with A21 = L21 D11 L11^T: the row y of
 // Y = A21 L11^-T solves y L11^T = a by unit forward substitution against L11, then
 // U12 = Y^T (= D11 L21^T) and L21 = Y D11^-1.
-// Stage: L11's strictly lower part, packed by rows (sL, 64 * 63 / 2), and D11 (sD, 64).
+// Stage: L11's strictly lower part, packed by rows (sL, 64 * 63 / 2), and the inverse of
+// D11 (sD, 64), so that L21 = Y D11^-1 multiplies as the diagonal block forms L11.
 template <class T>
 __device__ __forceinline__ void ldlt_rows_stage(const cx<T>* __restrict__ M, int np, int k0, cx<T>* sL, cx<T>* sD) {
     for (int e = threadIdx.x; e < 64 * 64; e += 256) {
         const int r = e & 63, c = e >> 6;
         if (r > c) sL[(r * (r - 1) >> 1) + c] = M[(size_t)(k0 + r) + (size_t)(k0 + c) * np];
-        else if (r == c) sD[r] = M[(size_t)(k0 + r) + (size_t)(k0 + r) * np];
+        else if (r == c) sD[r] = inv(M[(size_t)(k0 + r) + (size_t)(k0 + r) * np]);
     }
 }
 
@@ -172,7 +173,7 @@ __device__ __forceinline__ T ldlt_rows_solve(cx<T>* __restrict__ M, int np, int 
 #pragma unroll
         for (int m = 0; m < 16; m++) {
             const int i = 4 * m + qq;
-            const cx<T> l = xv[m] / sD[i];
+            const cx<T> l = xv[m] * sD[i];
             M[(size_t)(k0 + i) + row * np] = xv[m];                                   // U12 = D11 L21^T
             M[row + (size_t)(k0 + i) * np] = l;                                       // L21
             const T a = growth_of(l);
